@@ -1,0 +1,138 @@
+/* materialite_amd -- C ABI of the MI355X SmallStrainFFT solver.
+ *
+ * The reference (nasa/materialite v0.1.3) is pure Python and has no FFI; its
+ * boundary for this path is the Python API
+ *   SmallStrainFFT(load_schedule, ...)(material, ...)
+ *   (materialite/models/small_strain_fft/small_strain_fft.py:29-264).
+ * Each entry point below replaces the reference code cited beside it; the
+ * Python package materialite_amd binds them with ctypes and keeps the
+ * reference's Python API on top.
+ *
+ * Conventions: every function returns 0 on success and a negative status on
+ * failure, with the message in mtl_last_error() (thread-local).  Host arrays
+ * are plain C arrays; "dev" pointers are device (HBM) addresses, e.g. a torch
+ * tensor's data_ptr().  Point order is x slowest, z fastest
+ * (materialite/util/util.py:40-42).  Mandel order [xx, yy, zz, r2*yz, r2*xz,
+ * r2*xy] (materialite/basis_operations.py:37-48).  A handle is used by one host
+ * thread at a time; separate handles are independent.
+ */
+#ifndef MATERIALITE_AMD_H
+#define MATERIALITE_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mtl_solver mtl_solver;
+
+/* precision of a solver handle */
+enum { MTL_FP64 = 0, /* every field, FFT and Krylov vector in FP64 */
+       MTL_MIXED = 1 /* Krylov vectors, tangents and matvec FFTs in FP32;
+                        strain/stress, residual FFT and all norms in FP64 */ };
+
+/* hardening laws (small_strain_fft/hardening_laws.py) */
+enum { MTL_HARDENING_PERFECT = 0, /* :18-25 */
+       MTL_HARDENING_LINEAR = 1,  /* :28-39, props[0] = hardening_rate */
+       MTL_HARDENING_VOCE = 2,    /* :42-72, props = {theta_0, theta_1, tau_1} */
+       MTL_HARDENING_ARMSTRONG_FREDERICK = 3 /* :75-90, props = {direct_hardening,
+                                                dynamic_recovery} */ };
+
+/* fields readable with mtl_solver_get_field (AoS, point-major) */
+enum { MTL_FIELD_STRESS = 0,            /* 6 per point */
+       MTL_FIELD_STRAIN = 1,            /* 6 */
+       MTL_FIELD_PLASTIC_STRAIN = 2,    /* 6, committed, specimen frame */
+       MTL_FIELD_SLIP_RESISTANCE = 3,   /* nslip */
+       MTL_FIELD_SLIP_SHEAR = 4,        /* nslip */
+       MTL_FIELD_ACCUMULATED_SLIP = 5,  /* 1 */
+       MTL_FIELD_TANGENT = 6            /* 36 (full Mandel 6x6) */ };
+
+const char* mtl_last_error(void);
+
+/* Solver for one periodic grid.  Replaces the per-run setup of
+ * SmallStrainFFT.run (small_strain_fft.py:72-104) including
+ * _get_projection_operator (:301-368), which is never materialised here. */
+int mtl_solver_create(int nx, int ny, int nz, int precision, mtl_solver** out);
+int mtl_solver_destroy(mtl_solver* h);
+
+/* Constitutive models (crystal-frame Mandel 6x6 stiffness, row-major).
+ * Elastic: small_strain_fft/elastic.py:16-49. */
+int mtl_solver_add_elastic(mtl_solver* h, const double stiffness[36], int* model_id);
+/* ElasticViscoplastic: small_strain_fft/elastic_viscoplastic.py:20-294.
+ * schmid: num_slip x 6 Mandel sym(d (x) n) (slip_system.py:139-141). */
+int mtl_solver_add_evp(mtl_solver* h, const double stiffness[36], int num_slip,
+                       const double* schmid, double reference_slip_rate,
+                       double rate_exponent, double slip_resistance, int hardening,
+                       const double hardening_props[4], int* model_id);
+
+/* Microstructure: per-point rotation matrices (N x 9, v_crystal = R v_specimen,
+ * tensor.py:1855-1896) and per-point model ids (N int32; Multiphase dispatch,
+ * small_strain_fft.py:384-398 and multiphase.py:18-51). */
+int mtl_solver_set_microstructure(mtl_solver* h, const double* rotations,
+                                  const int32_t* model_of_point);
+
+/* Initial tangent and zeroed state (constitutive initialize(), elastic.py:23-27,
+ * elastic_viscoplastic.py:52-89; fields of small_strain_fft.py:93-104). */
+int mtl_solver_initialize(mtl_solver* h);
+
+typedef struct {
+  double strain_increment[6]; /* load_schedule.strain_increment(t, dt) */
+  double stress_increment[6]; /* load_schedule.stress_increment(t, dt) */
+  double applied_stress[6];   /* load_schedule.stress(t, dt) */
+  double stress_mask[6];      /* load_schedule.stress_mask */
+  double time_increment;
+  double global_tolerance;
+  double strain_correction_tolerance;
+  double linear_solver_tolerance;
+  double summed_von_mises_stress;
+  int time_step_id;
+  int reset_fluctuation; /* load reversal (small_strain_fft.py:111-116) */
+} mtl_step_params;
+
+typedef struct {
+  int converged;
+  int constitutive_failed;
+  int newton_iterations;
+  long long krylov_iterations;
+  int max_constitutive_iterations;
+  double mean_von_mises_stress;
+  double equilibrium_error;
+  double scaled_strain_error;
+  double projected_scaled_strain_error;
+} mtl_step_report;
+
+/* One load increment: predictor, Newton loop with MINRES corrections and
+ * convergence tests (small_strain_fft.py:106-211). */
+int mtl_solver_step(mtl_solver* h, const mtl_step_params* p, mtl_step_report* r);
+/* Accept the increment (small_strain_fft.py:213-241): old fields <- new,
+ * constitutive commit, fluctuation strain scaled by dt_new/dt. */
+int mtl_solver_commit(mtl_solver* h, double fluctuation_scale);
+/* Cut back (small_strain_fft.py:242-246): fluctuation strain halved. */
+int mtl_solver_rollback(mtl_solver* h);
+/* Copy a field to a host array (N x ncomp). ncomp receives the width. */
+int mtl_solver_get_field(mtl_solver* h, int field, double* out, int* ncomp);
+
+/* ---- kernel-level entry points (device pointers, stream-ordered) ---- */
+
+/* out = F^-1[Ghat . F(in)] (real part) for a [6][N] SoA FP64 field, computed
+ * in the handle's FFT precision (_apply_projection_tensor,
+ * small_strain_fft.py:376-382). */
+int mtl_solver_project(mtl_solver* h, const double* in_dev, double* out_dev,
+                       const double stress_mask[6]);
+/* MINRES solve of Ghat*(D:x) = b with the handle's current tangent
+ * (_get_fluctuation_strain, small_strain_fft.py:281-299); b, x are [6][N]
+ * FP64 device arrays. */
+int mtl_solver_minres(mtl_solver* h, const double* b_dev, double* x_dev,
+                      const double stress_mask[6], double rtol, int* iterations);
+/* Constitutive update of every model (calculate_stress_and_tangent) on
+ * [6][N] FP64 device strain and guess; stress is written to stress_dev and the
+ * tangent to the handle.  iterations = 0 on non-convergence. */
+int mtl_solver_constitutive(mtl_solver* h, const double* strain_dev, const double* guess_dev,
+                            double time_increment, double* stress_dev, int* iterations);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MATERIALITE_AMD_H */

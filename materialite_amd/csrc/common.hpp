@@ -1,0 +1,101 @@
+// Shared device/host helpers for the materialite_amd HIP library (gfx950).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <stdexcept>
+#include <string>
+
+namespace mtl {
+
+// ------------------------------------------------------------------ errors
+struct Error : std::runtime_error {
+  explicit Error(const std::string& m) : std::runtime_error(m) {}
+};
+
+#define MTL_HIP(call)                                                          \
+  do {                                                                         \
+    hipError_t e_ = (call);                                                    \
+    if (e_ != hipSuccess)                                                      \
+      throw ::mtl::Error(std::string("HIP error '") + hipGetErrorString(e_) +  \
+                         "' at " + __FILE__ + ":" + std::to_string(__LINE__)); \
+  } while (0)
+
+#define MTL_LAUNCH_CHECK() MTL_HIP(hipGetLastError())
+
+// ------------------------------------------------------------------ complex
+template <typename T>
+struct cplx {
+  T x, y;
+};
+
+template <typename T>
+__device__ __forceinline__ cplx<T> cmul(cplx<T> a, cplx<T> b) {
+  return {a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cadd(cplx<T> a, cplx<T> b) {
+  return {a.x + b.x, a.y + b.y};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> csub(cplx<T> a, cplx<T> b) {
+  return {a.x - b.x, a.y - b.y};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cscale(cplx<T> a, T s) {
+  return {a.x * s, a.y * s};
+}
+template <typename T>
+__device__ __forceinline__ cplx<T> cconj(cplx<T> a) {
+  return {a.x, -a.y};
+}
+
+// ------------------------------------------------------------------ launch shape
+// MI355X dispatches workgroups round-robin over its 8 XCDs (each with its own
+// L2).  Remapping blockIdx so that XCD j receives a contiguous range of logical
+// blocks keeps neighbouring tiles (which share cache lines) in one L2.
+__device__ __forceinline__ int xcd_logical_block(int b, int nblocks) {
+  const int nx = 8;
+  if (nblocks % nx != 0) return b;
+  return (b % nx) * (nblocks / nx) + b / nx;
+}
+
+// block-wide sum of a double, result valid in thread 0
+template <int BT>
+__device__ __forceinline__ double block_sum(double v, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int i = 0; i < BT / 64; ++i) r += red[i];
+  }
+  __syncthreads();
+  return r;
+}
+
+template <int BT>
+__device__ __forceinline__ double block_max(double v, double* red) {
+  const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_down(v, o, 64));
+  if (lane == 0) red[w] = v;
+  __syncthreads();
+  double r = 0.0;
+  if (threadIdx.x == 0) {
+    r = red[0];
+#pragma unroll
+    for (int i = 1; i < BT / 64; ++i) r = fmax(r, red[i]);
+  }
+  __syncthreads();
+  return r;
+}
+
+inline int ceil_div(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
+
+}  // namespace mtl

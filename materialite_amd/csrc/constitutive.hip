@@ -1,0 +1,781 @@
+// Elastic and elasto-viscoplastic (crystal plasticity) updates on gfx950.
+//
+// Elastic (small_strain_fft/elastic.py:23-31): sigma = Rm^T C Rm : eps.
+//
+// Elasto-viscoplastic (small_strain_fft/elastic_viscoplastic.py:91-271), per
+// point in the crystal frame, one implicit Newton step per iteration:
+//   tau_s = P_s : sig,  pref_s = g0dot |tau_s/g_s|^(n-1) / g_s,  rate_s = pref_s tau_s
+//   ep    = ep_old + dt sum_s rate_s P_s
+//   J     = I + dt C M,  M = sum_s n pref_s P_s (x) P_s
+//   sig  <- sig - J^-1 (sig - C (eps - ep)),   g <- hardening(...)
+// Since C is SPD and M is PSD, J^-1 = S^-1 C^-1 with S = C^-1 + dt M SPD, so the
+// update is computed as sig - S^-1 (C^-1 sig - (eps - ep)) with a Cholesky
+// inverse of S, and the consistent tangent J^-1 C equals S^-1 (symmetric).
+//
+// Iteration order follows the reference exactly: every point iterates until at
+// least half of the model's points satisfy the convergence test (iteration
+// K*), after which only the unconverged points iterate, each independently, up
+// to 60 iterations in total.  The unmasked phase runs one kernel per iteration
+// with the state in HBM (ping-pong buffers and a device converged-count); the
+// masked phase and the outputs run in one kernel with the state in registers.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+
+#include "constitutive.hpp"
+
+namespace mtl {
+
+namespace {
+
+constexpr int kCBT = 256;
+
+// Mandel form of a rotation R (v_crystal = R v_specimen):
+// (R A R^T) in Mandel = Rm A_mandel  (tensor.py:1668-1700).
+__device__ __forceinline__ void mandel_rotation(const double* R, double Rm[36]) {
+  const double r2 = 1.41421356237309504880168872420969807857;
+#define R_(i, j) R[(i)*3 + (j)]
+#pragma unroll
+  for (int i = 0; i < 3; ++i)
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Rm[i * 6 + j] = R_(i, j) * R_(i, j);
+#pragma unroll
+  for (int i = 0; i < 3; ++i) {
+    Rm[i * 6 + 3] = r2 * R_(i, 1) * R_(i, 2);
+    Rm[i * 6 + 4] = r2 * R_(i, 0) * R_(i, 2);
+    Rm[i * 6 + 5] = r2 * R_(i, 0) * R_(i, 1);
+  }
+  // rows 3,4,5 pair the (1,2), (0,2), (0,1) rows of R
+  const int pa[3] = {1, 0, 0}, pb[3] = {2, 2, 1};
+#pragma unroll
+  for (int m = 0; m < 3; ++m) {
+    const int a = pa[m], b = pb[m];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) Rm[(3 + m) * 6 + j] = r2 * R_(a, j) * R_(b, j);
+    Rm[(3 + m) * 6 + 3] = R_(a, 1) * R_(b, 2) + R_(a, 2) * R_(b, 1);
+    Rm[(3 + m) * 6 + 4] = R_(a, 0) * R_(b, 2) + R_(a, 2) * R_(b, 0);
+    Rm[(3 + m) * 6 + 5] = R_(a, 0) * R_(b, 1) + R_(a, 1) * R_(b, 0);
+  }
+#undef R_
+}
+
+__device__ __forceinline__ void load_rotation(const double* rot9, long long npts, long long p,
+                                              double Rm[36]) {
+  double R[9];
+#pragma unroll
+  for (int k = 0; k < 9; ++k) R[k] = rot9[k * npts + p];
+  mandel_rotation(R, Rm);
+}
+
+// v_c = Rm v_s
+__device__ __forceinline__ void to_crystal(const double Rm[36], const double vs[6], double vc[6]) {
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) acc += Rm[a * 6 + b] * vs[b];
+    vc[a] = acc;
+  }
+}
+
+// v_s = Rm^T v_c
+__device__ __forceinline__ void to_specimen(const double Rm[36], const double vc[6], double vs[6]) {
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) acc += Rm[b * 6 + a] * vc[b];
+    vs[a] = acc;
+  }
+}
+
+// Ds = Rm^T Dc Rm for a packed symmetric Dc  (tensor.py:1620-1634)
+__device__ __forceinline__ void tangent_to_specimen(const double Rm[36], const double Dc[21],
+                                                    double Ds[21]) {
+  double T[36];  // Dc Rm
+#pragma unroll
+  for (int i = 0; i < 6; ++i)
+#pragma unroll
+    for (int b = 0; b < 6; ++b) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc += Dc[sym6(i, j)] * Rm[j * 6 + b];
+      T[i * 6 + b] = acc;
+    }
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = a; b < 6; ++b) {
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) acc += Rm[i * 6 + a] * T[i * 6 + b];
+      Ds[sym6(a, b)] = acc;
+    }
+}
+
+template <typename K>
+__device__ __forceinline__ void store_tangent(K* tangent, long long npts, long long p,
+                                              const double Ds[21]) {
+#pragma unroll
+  for (int k = 0; k < 21; ++k) tangent[k * npts + p] = K(Ds[k]);
+}
+
+struct Stiff21 {
+  double c[21];
+};
+
+// ------------------------------------------------------------------ elastic
+template <typename K>
+__global__ void __launch_bounds__(kCBT) k_elastic(int n, const int* idx, long long npts,
+                                                   const double* rot9, Stiff21 C,
+                                                   const double* eps, double* sig, K* tangent,
+                                                   int write_stress) {
+  const int i = blockIdx.x * kCBT + threadIdx.x;
+  if (i >= n) return;
+  const long long p = idx[i];
+  double Rm[36], Ds[21];
+  load_rotation(rot9, npts, p, Rm);
+  tangent_to_specimen(Rm, C.c, Ds);
+  store_tangent(tangent, npts, p, Ds);
+  if (write_stress) {
+    double e[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) e[c] = eps[c * npts + p];
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      double acc = 0.0;
+#pragma unroll
+      for (int b = 0; b < 6; ++b) acc += Ds[sym6(a, b)] * e[b];
+      sig[a * npts + p] = acc;
+    }
+  }
+}
+
+// ------------------------------------------------------------------ EVP
+template <int NS>
+struct EvpParams {
+  double P[NS][6];
+  double Cinv[21];
+  double gamma0, n, dt, ref_modulus, tol;
+  int nint;  // n-1 when it is an integer in [0, 64], else -1
+  int hard;
+  double hp[4];
+};
+
+struct EvpControl {
+  int count[kEvpMaxIterations + 4];
+  int stopped;
+  int kstar;
+  int fail;
+  int extra;
+};
+
+__device__ __forceinline__ double powi_abs(double x, int e, double ef) {
+  x = fabs(x);
+  if (e < 0) return pow(x, ef);
+  double r = 1.0, b = x;
+  while (e) {
+    if (e & 1) r *= b;
+    b *= b;
+    e >>= 1;
+  }
+  return r;
+}
+
+// Cholesky-based inverse of a packed SPD 6x6.
+__device__ __forceinline__ void spd_inverse6(const double S[21], double Sinv[21]) {
+  double L[6][6];
+#pragma unroll
+  for (int j = 0; j < 6; ++j) {
+    double d = S[sym6(j, j)];
+#pragma unroll
+    for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
+    const double ljj = sqrt(d);
+    L[j][j] = ljj;
+    const double il = 1.0 / ljj;
+#pragma unroll
+    for (int i = j + 1; i < 6; ++i) {
+      double v = S[sym6(i, j)];
+#pragma unroll
+      for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
+      L[i][j] = v * il;
+    }
+  }
+  double Li[6][6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    Li[i][i] = 1.0 / L[i][i];
+#pragma unroll
+    for (int j = 0; j < i; ++j) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = j; k < i; ++k) acc += L[i][k] * Li[k][j];
+      Li[i][j] = -acc * Li[i][i];
+    }
+  }
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = a; b < 6; ++b) {
+      double acc = 0.0;
+#pragma unroll
+      for (int k = b; k < 6; ++k) acc += Li[k][a] * Li[k][b];
+      Sinv[sym6(a, b)] = acc;
+    }
+}
+
+// One iteration of the per-point implicit update (elastic_viscoplastic.py:190-253).
+template <int NS>
+__device__ __forceinline__ void evp_iterate(const EvpParams<NS>& q, const double sig[6],
+                                            const double g[NS], const double eps[6],
+                                            const double ep_old[6], const double g_old[NS],
+                                            double acc_old, double sig_new[6], double g_new[NS],
+                                            double ep[6], double dgam[NS], double Sinv[21],
+                                            double& diff) {
+  double rate[NS], w[NS];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    double tau = 0.0;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) tau += q.P[s][m] * sig[m];
+    const double pref = q.gamma0 * powi_abs(tau / g[s], q.nint, q.n - 1.0) / g[s];
+    rate[s] = pref * tau;
+    w[s] = q.n * pref;
+  }
+#pragma unroll
+  for (int m = 0; m < 6; ++m) {
+    double acc = 0.0;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) acc += rate[s] * q.P[s][m];
+    ep[m] = ep_old[m] + acc * q.dt;
+  }
+  double S[21];
+#pragma unroll
+  for (int a = 0; a < 6; ++a)
+#pragma unroll
+    for (int b = a; b < 6; ++b) {
+      double acc = 0.0;
+#pragma unroll
+      for (int s = 0; s < NS; ++s) acc += w[s] * q.P[s][a] * q.P[s][b];
+      S[sym6(a, b)] = q.Cinv[sym6(a, b)] + q.dt * acc;
+    }
+  spd_inverse6(S, Sinv);
+  double rc[6];
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) acc += q.Cinv[sym6(a, b)] * sig[b];
+    rc[a] = acc - (eps[a] - ep[a]);
+  }
+  double d2 = 0.0;
+#pragma unroll
+  for (int a = 0; a < 6; ++a) {
+    double acc = 0.0;
+#pragma unroll
+    for (int b = 0; b < 6; ++b) acc += Sinv[sym6(a, b)] * rc[b];
+    sig_new[a] = sig[a] - acc;
+    d2 += acc * acc;
+  }
+  diff = sqrt(d2);
+  // hardening (hardening_laws.py:18-90): one slip increment for all systems
+  double dsum = 0.0;
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    dgam[s] = rate[s] * q.dt;
+    dsum += fabs(rate[s]);
+  }
+  const double d = q.dt * dsum;
+  if (q.hard == kLinear) {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) g_new[s] = g_old[s] + q.hp[0] * d;
+  } else if (q.hard == kVoce) {
+    const double theta0 = q.hp[0], theta1 = q.hp[1], tau1 = q.hp[2];
+    const double ratio = fabs(theta0 / tau1);
+    const double e0 = exp(-ratio * acc_old);
+    const double ed = exp(-ratio * d);
+    const double inc = theta1 * d - (ratio * tau1 - theta1) / ratio * e0 * (ed - 1.0) -
+                       theta1 / ratio * e0 * (ed * (1.0 + ratio * (acc_old + d)) - 1.0 - ratio * acc_old);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) g_new[s] = g_old[s] + inc;
+  } else if (q.hard == kArmstrongFrederick) {
+    const double h = q.hp[0], r = q.hp[1];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) g_new[s] = g_old[s] + (h - r * g_old[s]) * d / (1.0 + r * d);
+  } else {
+#pragma unroll
+    for (int s = 0; s < NS; ++s) g_new[s] = g_old[s];
+  }
+}
+
+template <int NS>
+__device__ __forceinline__ bool evp_converged(const EvpParams<NS>& q, double diff, double gnorm) {
+  // elastic_viscoplastic.py:160-164 (a 0/0 or x/0 ratio compares false)
+  return (diff / gnorm <= q.tol) || (diff / q.ref_modulus < 1.0e-12);
+}
+
+struct EvpBuffers {
+  double* eps_c;     // [6][n]
+  double* gnorm;     // [n]
+  double* sig[2];    // [6][n] ping-pong
+  double* g[2];      // [NS][n] ping-pong
+  unsigned char* conv;
+  double* ep_old;    // [6][n]
+  double* g_old;     // [NS][n]
+  double* gam_old;   // [NS][n]
+  double* acc_old;   // [n]
+  double* ep_pend;   // outputs of the call, adopted as trial state on success
+  double* g_pend;
+  double* gam_pend;
+  double* acc_pend;
+};
+
+template <int NS>
+__global__ void __launch_bounds__(kCBT) k_evp_setup(int n, const int* idx, long long npts,
+                                                     const double* rot9, const double* eps,
+                                                     const double* guess, const double* g_trial,
+                                                     EvpBuffers B) {
+  const int i = blockIdx.x * kCBT + threadIdx.x;
+  if (i >= n) return;
+  const long long p = idx[i];
+  double Rm[36], v[6], vc[6];
+  load_rotation(rot9, npts, p, Rm);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) v[c] = eps[c * npts + p];
+  to_crystal(Rm, v, vc);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) B.eps_c[c * (long long)n + i] = vc[c];
+  double nrm = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    v[c] = guess[c * npts + p];
+    nrm += v[c] * v[c];
+  }
+  B.gnorm[i] = sqrt(nrm);
+  to_crystal(Rm, v, vc);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) B.sig[0][c * (long long)n + i] = vc[c];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) B.g[0][s * (long long)n + i] = g_trial[s * (long long)n + i];
+}
+
+__global__ void k_evp_ctrl_reset(EvpControl* c) {
+  int* w = reinterpret_cast<int*>(c);
+  for (int i = threadIdx.x; i < int(sizeof(EvpControl) / sizeof(int)); i += blockDim.x) w[i] = 0;
+}
+
+template <int NS>
+__device__ __forceinline__ void load_point(const EvpBuffers& B, int n, int i, int buf, double sig[6],
+                                           double g[NS], double eps[6], double ep_old[6],
+                                           double g_old[NS], double& acc_old) {
+  const long long ln = n;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    sig[c] = B.sig[buf][c * ln + i];
+    eps[c] = B.eps_c[c * ln + i];
+    ep_old[c] = B.ep_old[c * ln + i];
+  }
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    g[s] = B.g[buf][s * ln + i];
+    g_old[s] = B.g_old[s * ln + i];
+  }
+  acc_old = B.acc_old[i];
+}
+
+// unmasked iteration `it` (1-based): reads buffer (it-1)&1, writes it&1
+template <int NS>
+__global__ void __launch_bounds__(kCBT) k_evp_iter_all(int it, int n, EvpParams<NS> q,
+                                                        EvpBuffers B, EvpControl* ctrl) {
+  if (ctrl->stopped) return;
+  __shared__ int cnt;
+  if (threadIdx.x == 0) cnt = 0;
+  __syncthreads();
+  const int i = blockIdx.x * kCBT + threadIdx.x;
+  const int src = (it - 1) & 1, dst = it & 1;
+  if (i < n) {
+    double sig[6], g[NS], eps[6], ep_old[6], g_old[NS], acc_old;
+    load_point<NS>(B, n, i, src, sig, g, eps, ep_old, g_old, acc_old);
+    double sig_new[6], g_new[NS], ep[6], dgam[NS], Sinv[21], diff;
+    evp_iterate<NS>(q, sig, g, eps, ep_old, g_old, acc_old, sig_new, g_new, ep, dgam, Sinv, diff);
+    const long long ln = n;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) B.sig[dst][c * ln + i] = sig_new[c];
+#pragma unroll
+    for (int s = 0; s < NS; ++s) B.g[dst][s * ln + i] = g_new[s];
+    const bool cv = evp_converged<NS>(q, diff, B.gnorm[i]);
+    B.conv[i] = cv ? 1 : 0;
+    if (cv) atomicAdd(&cnt, 1);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0 && cnt) atomicAdd(&ctrl->count[it], cnt);
+}
+
+__global__ void k_evp_control(int it, int n, EvpControl* ctrl) {
+  if (threadIdx.x != 0 || ctrl->stopped) return;
+  const long long c = ctrl->count[it];
+  if (2 * c >= n) {  // converged fraction >= 0.5 (elastic_viscoplastic.py:116)
+    ctrl->kstar = it;
+    ctrl->stopped = 1;
+  } else if (it >= kEvpMaxIterations) {
+    ctrl->fail = 1;
+    ctrl->stopped = 1;
+  }
+}
+
+template <int NS, typename K>
+__global__ void __launch_bounds__(kCBT) k_evp_finish(int n, const int* idx, long long npts,
+                                                      const double* rot9, EvpParams<NS> q,
+                                                      EvpBuffers B, EvpControl* ctrl, double* sig_out,
+                                                      K* tangent) {
+  const int i = blockIdx.x * kCBT + threadIdx.x;
+  if (i >= n || ctrl->fail) return;
+  const int kstar = ctrl->kstar;
+  const int after = kstar & 1, before = (kstar - 1) & 1;
+  double sig[6], g[NS], eps[6], ep_old[6], g_old[NS], acc_old;
+  double sig_new[6], g_new[NS], ep[6], dgam[NS], Sinv[21], diff;
+  int extra = 0;
+  if (B.conv[i]) {
+    // converged at K*: recompute iteration K* from its starting state to get
+    // its plastic strain, slip increments and tangent (bitwise the same sig/g)
+    load_point<NS>(B, n, i, before, sig, g, eps, ep_old, g_old, acc_old);
+    evp_iterate<NS>(q, sig, g, eps, ep_old, g_old, acc_old, sig_new, g_new, ep, dgam, Sinv, diff);
+  } else {
+    load_point<NS>(B, n, i, after, sig, g, eps, ep_old, g_old, acc_old);
+    const double gn = B.gnorm[i];
+    bool cv = false;
+    for (int it = kstar + 1; it <= kEvpMaxIterations; ++it) {
+      evp_iterate<NS>(q, sig, g, eps, ep_old, g_old, acc_old, sig_new, g_new, ep, dgam, Sinv, diff);
+      ++extra;
+      cv = evp_converged<NS>(q, diff, gn);
+      if (cv) break;
+#pragma unroll
+      for (int c = 0; c < 6; ++c) sig[c] = sig_new[c];
+#pragma unroll
+      for (int s = 0; s < NS; ++s) g[s] = g_new[s];
+    }
+    if (!cv) {
+      atomicOr(&ctrl->fail, 1);
+      return;
+    }
+    atomicMax(&ctrl->extra, extra);
+  }
+  const long long p = idx[i];
+  const long long ln = n;
+  double Rm[36], vs[6], Ds[21];
+  load_rotation(rot9, npts, p, Rm);
+  to_specimen(Rm, sig_new, vs);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) sig_out[c * npts + p] = vs[c];
+  tangent_to_specimen(Rm, Sinv, Ds);
+  store_tangent(tangent, npts, p, Ds);
+  double asum = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) B.ep_pend[c * ln + i] = ep[c];
+#pragma unroll
+  for (int s = 0; s < NS; ++s) {
+    B.g_pend[s * ln + i] = g_new[s];
+    B.gam_pend[s * ln + i] = B.gam_old[s * ln + i] + dgam[s];
+    asum += fabs(dgam[s]);
+  }
+  B.acc_pend[i] = acc_old + asum;
+}
+
+// crystal plastic strain -> specimen frame, AoS output for the host
+__global__ void __launch_bounds__(kCBT) k_ep_specimen(int n, const int* idx, long long npts,
+                                                       const double* rot9, const double* ep,
+                                                       double* out) {
+  const int i = blockIdx.x * kCBT + threadIdx.x;
+  if (i >= n) return;
+  double Rm[36], vc[6], vs[6];
+  load_rotation(rot9, npts, idx[i], Rm);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) vc[c] = ep[c * (long long)n + i];
+  to_specimen(Rm, vc, vs);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) out[i * 6LL + c] = vs[c];
+}
+
+__global__ void k_fill(double* a, long long count, double v) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x)
+    a[i] = v;
+}
+
+void fill(double* a, long long count, double v, hipStream_t s) {
+  const int blocks = int(std::min<long long>((count + 255) / 256, 4096));
+  hipLaunchKernelGGL(k_fill, dim3(std::max(blocks, 1)), dim3(256), 0, s, a, count, v);
+  MTL_LAUNCH_CHECK();
+}
+
+Stiff21 pack21(const double* C36) {
+  Stiff21 s{};
+  for (int i = 0; i < 6; ++i)
+    for (int j = i; j < 6; ++j) s.c[sym6(i, j)] = 0.5 * (C36[i * 6 + j] + C36[j * 6 + i]);
+  return s;
+}
+
+// 6x6 inverse with partial pivoting (host)
+bool inverse6(const double* A, double* out) {
+  double m[6][12];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 12; ++j) m[i][j] = j < 6 ? A[i * 6 + j] : (j - 6 == i ? 1.0 : 0.0);
+  for (int c = 0; c < 6; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 6; ++r)
+      if (std::fabs(m[r][c]) > std::fabs(m[piv][c])) piv = r;
+    if (m[piv][c] == 0.0) return false;
+    if (piv != c)
+      for (int j = 0; j < 12; ++j) std::swap(m[c][j], m[piv][j]);
+    const double inv = 1.0 / m[c][c];
+    for (int j = 0; j < 12; ++j) m[c][j] *= inv;
+    for (int r = 0; r < 6; ++r)
+      if (r != c) {
+        const double f = m[r][c];
+        if (f != 0.0)
+          for (int j = 0; j < 12; ++j) m[r][j] -= f * m[c][j];
+      }
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) out[i * 6 + j] = m[i][j + 6];
+  return true;
+}
+
+int* upload_idx(const std::vector<int>& v) {
+  int* d = nullptr;
+  MTL_HIP(hipMalloc(&d, std::max<size_t>(v.size(), 1) * sizeof(int)));
+  if (!v.empty()) MTL_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(int), hipMemcpyHostToDevice));
+  return d;
+}
+
+// ------------------------------------------------------------------ models
+template <typename K>
+class ElasticModel final : public Model<K> {
+ public:
+  ElasticModel(const ModelDesc& d, const std::vector<int>& pts, long long npts) {
+    this->n_ = int(pts.size());
+    this->npts_ = npts;
+    this->host_idx_ = pts;
+    this->idx_ = upload_idx(pts);
+    C_ = pack21(d.C);
+  }
+  ~ElasticModel() override { (void)hipFree(this->idx_); }
+  void init_tangent(const double* rot9, K* tangent, hipStream_t s) override {
+    launch(rot9, nullptr, nullptr, tangent, 0, s);
+  }
+  int calculate(const double* rot9, const double* eps, const double*, double, double* sig,
+                K* tangent, hipStream_t s) override {
+    launch(rot9, eps, sig, tangent, 1, s);
+    return 1;
+  }
+  void commit(const double*, hipStream_t) override {}
+  bool field(int, const double*, std::vector<double>&, int&, hipStream_t) override { return false; }
+
+ private:
+  void launch(const double* rot9, const double* eps, double* sig, K* tangent, int ws,
+              hipStream_t s) {
+    if (this->n_ == 0) return;
+    hipLaunchKernelGGL(k_elastic<K>, dim3(ceil_div(this->n_, kCBT)), dim3(kCBT), 0, s, this->n_,
+                       this->idx_, this->npts_, rot9, C_, eps, sig, tangent, ws);
+    MTL_LAUNCH_CHECK();
+  }
+  Stiff21 C_;
+};
+
+template <typename K, int NS>
+class EvpModel final : public Model<K> {
+ public:
+  EvpModel(const ModelDesc& d, const std::vector<int>& pts, long long npts) : d_(d) {
+    this->n_ = int(pts.size());
+    this->npts_ = npts;
+    this->host_idx_ = pts;
+    this->idx_ = upload_idx(pts);
+    C_ = pack21(d.C);
+    double Cinv[36];
+    if (!inverse6(d.C, Cinv)) throw Error("elasto-viscoplastic stiffness must be invertible");
+    // reference modulus E[100] = 1 / (e1 e1 : C^-1 : e1 e1)  (elastic_viscoplastic.py:32-34)
+    q_.ref_modulus = 1.0 / Cinv[0];
+    const Stiff21 ci = pack21(Cinv);
+    for (int k = 0; k < 21; ++k) q_.Cinv[k] = ci.c[k];
+    for (int s = 0; s < NS; ++s)
+      for (int m = 0; m < 6; ++m) q_.P[s][m] = d.schmid[s * 6 + m];
+    q_.gamma0 = d.gamma0;
+    q_.n = d.n;
+    const double e = d.n - 1.0;
+    q_.nint = (e >= 0.0 && e <= 64.0 && std::floor(e) == e) ? int(e) : -1;
+    q_.tol = kEvpTolerance;
+    q_.hard = d.hardening;
+    for (int k = 0; k < 4; ++k) q_.hp[k] = d.hp[k];
+    const long long n = std::max(this->n_, 1);
+    auto alloc = [&](double*& ptr, long long cnt) { MTL_HIP(hipMalloc(&ptr, cnt * sizeof(double))); };
+    alloc(B_.eps_c, 6 * n);
+    alloc(B_.gnorm, n);
+    for (int k = 0; k < 2; ++k) {
+      alloc(B_.sig[k], 6 * n);
+      alloc(B_.g[k], NS * n);
+    }
+    MTL_HIP(hipMalloc(&B_.conv, n));
+    alloc(B_.ep_old, 6 * n);
+    alloc(B_.g_old, NS * n);
+    alloc(B_.gam_old, NS * n);
+    alloc(B_.acc_old, n);
+    alloc(B_.ep_pend, 6 * n);
+    alloc(B_.g_pend, NS * n);
+    alloc(B_.gam_pend, NS * n);
+    alloc(B_.acc_pend, n);
+    alloc(ep_tr_, 6 * n);
+    alloc(g_tr_, NS * n);
+    alloc(gam_tr_, NS * n);
+    alloc(acc_tr_, n);
+    MTL_HIP(hipMalloc(&ctrl_, sizeof(EvpControl)));
+    MTL_HIP(hipHostMalloc(&hctrl_, sizeof(EvpControl), hipHostMallocDefault));
+  }
+  ~EvpModel() override {
+    for (double* p : {B_.eps_c, B_.gnorm, B_.sig[0], B_.sig[1], B_.g[0], B_.g[1], B_.ep_old,
+                      B_.g_old, B_.gam_old, B_.acc_old, B_.ep_pend, B_.g_pend, B_.gam_pend,
+                      B_.acc_pend, ep_tr_, g_tr_, gam_tr_, acc_tr_})
+      if (p) (void)hipFree(p);
+    if (B_.conv) (void)hipFree(B_.conv);
+    if (ctrl_) (void)hipFree(ctrl_);
+    if (hctrl_) (void)hipHostFree(hctrl_);
+    (void)hipFree(this->idx_);
+  }
+
+  void init_tangent(const double* rot9, K* tangent, hipStream_t s) override {
+    // elastic_viscoplastic.py:52-89: zero plastic state, g = g0
+    const long long n = this->n_;
+    if (n == 0) return;
+    fill(B_.ep_old, 6 * n, 0.0, s);
+    fill(ep_tr_, 6 * n, 0.0, s);
+    fill(B_.g_old, NS * n, d_.g0, s);
+    fill(g_tr_, NS * n, d_.g0, s);
+    fill(B_.gam_old, NS * n, 0.0, s);
+    fill(gam_tr_, NS * n, 0.0, s);
+    fill(B_.acc_old, n, 0.0, s);
+    fill(acc_tr_, n, 0.0, s);
+    hipLaunchKernelGGL(k_elastic<K>, dim3(ceil_div(this->n_, kCBT)), dim3(kCBT), 0, s, this->n_,
+                       this->idx_, this->npts_, rot9, C_, nullptr, nullptr, tangent, 0);
+    MTL_LAUNCH_CHECK();
+  }
+
+  int calculate(const double* rot9, const double* eps, const double* guess, double dt,
+                double* sig, K* tangent, hipStream_t s) override {
+    const int n = this->n_;
+    if (n == 0) return 1;
+    q_.dt = dt;
+    const int blocks = ceil_div(n, kCBT);
+    hipLaunchKernelGGL(k_evp_setup<NS>, dim3(blocks), dim3(kCBT), 0, s, n, this->idx_, this->npts_,
+                       rot9, eps, guess, g_tr_, B_);
+    MTL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_evp_ctrl_reset, dim3(1), dim3(64), 0, s, ctrl_);
+    MTL_LAUNCH_CHECK();
+    int it = 0;
+    const int chunk = 4;
+    while (true) {
+      for (int k = 0; k < chunk && it < kEvpMaxIterations; ++k) {
+        ++it;
+        hipLaunchKernelGGL(k_evp_iter_all<NS>, dim3(blocks), dim3(kCBT), 0, s, it, n, q_, B_,
+                           ctrl_);
+        MTL_LAUNCH_CHECK();
+        hipLaunchKernelGGL(k_evp_control, dim3(1), dim3(64), 0, s, it, n, ctrl_);
+        MTL_LAUNCH_CHECK();
+      }
+      MTL_HIP(hipMemcpyAsync(hctrl_, ctrl_, sizeof(EvpControl), hipMemcpyDeviceToHost, s));
+      MTL_HIP(hipStreamSynchronize(s));
+      if (hctrl_->stopped || it >= kEvpMaxIterations) break;
+    }
+    if (hctrl_->fail || !hctrl_->stopped) return 0;
+    hipLaunchKernelGGL((k_evp_finish<NS, K>), dim3(blocks), dim3(kCBT), 0, s, n, this->idx_,
+                       this->npts_, rot9, q_, B_, ctrl_, sig, tangent);
+    MTL_LAUNCH_CHECK();
+    MTL_HIP(hipMemcpyAsync(hctrl_, ctrl_, sizeof(EvpControl), hipMemcpyDeviceToHost, s));
+    MTL_HIP(hipStreamSynchronize(s));
+    if (hctrl_->fail) return 0;
+    // adopt the call's outputs as the trial state (elastic_viscoplastic.py:175-182)
+    std::swap(B_.ep_pend, ep_tr_);
+    std::swap(B_.g_pend, g_tr_);
+    std::swap(B_.gam_pend, gam_tr_);
+    std::swap(B_.acc_pend, acc_tr_);
+    return hctrl_->kstar + hctrl_->extra;
+  }
+
+  void commit(const double*, hipStream_t s) override {
+    // elastic_viscoplastic.py:255-271
+    const size_t n = size_t(this->n_);
+    if (n == 0) return;
+    MTL_HIP(hipMemcpyAsync(B_.ep_old, ep_tr_, 6 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    MTL_HIP(hipMemcpyAsync(B_.g_old, g_tr_, NS * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    MTL_HIP(hipMemcpyAsync(B_.gam_old, gam_tr_, NS * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    MTL_HIP(hipMemcpyAsync(B_.acc_old, acc_tr_, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+
+  bool field(int which, const double* rot9, std::vector<double>& out, int& ncomp,
+             hipStream_t s) override {
+    const int n = this->n_;
+    std::vector<double> soa;
+    auto fetch = [&](const double* src, int nc) {
+      soa.resize(size_t(nc) * n);
+      if (n) MTL_HIP(hipMemcpyAsync(soa.data(), src, soa.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+      MTL_HIP(hipStreamSynchronize(s));
+      out.resize(size_t(nc) * n);
+      for (int i = 0; i < n; ++i)
+        for (int c = 0; c < nc; ++c) out[size_t(i) * nc + c] = soa[size_t(c) * n + i];
+      ncomp = nc;
+    };
+    switch (which) {
+      case kFieldPlasticStrain: {
+        // committed crystal plastic strain, rotated to the specimen frame
+        double* tmp = nullptr;
+        MTL_HIP(hipMalloc(&tmp, std::max(6 * n, 1) * sizeof(double)));
+        if (n)
+          hipLaunchKernelGGL(k_ep_specimen, dim3(ceil_div(n, kCBT)), dim3(kCBT), 0, s, n,
+                             this->idx_, this->npts_, rot9, B_.ep_old, tmp);
+        out.resize(size_t(6) * n);
+        if (n) MTL_HIP(hipMemcpyAsync(out.data(), tmp, out.size() * sizeof(double),
+                                      hipMemcpyDeviceToHost, s));
+        MTL_HIP(hipStreamSynchronize(s));
+        (void)hipFree(tmp);
+        ncomp = 6;
+        return true;
+      }
+      case kFieldSlipResistance: fetch(g_tr_, NS); return true;
+      case kFieldSlipShear: fetch(gam_tr_, NS); return true;
+      case kFieldAccumulatedSlip: fetch(acc_tr_, 1); return true;
+      default: return false;
+    }
+  }
+
+ private:
+  ModelDesc d_;
+  Stiff21 C_;
+  EvpParams<NS> q_{};
+  EvpBuffers B_{};
+  double *ep_tr_ = nullptr, *g_tr_ = nullptr, *gam_tr_ = nullptr, *acc_tr_ = nullptr;
+  EvpControl* ctrl_ = nullptr;
+  EvpControl* hctrl_ = nullptr;
+};
+
+}  // namespace
+
+template <typename K>
+std::unique_ptr<Model<K>> make_model(const ModelDesc& d, const std::vector<int>& points,
+                                     long long npts) {
+  if (d.kind == 0) return std::make_unique<ElasticModel<K>>(d, points, npts);
+  switch (d.nslip) {
+    case 12: return std::make_unique<EvpModel<K, 12>>(d, points, npts);
+    case 1: return std::make_unique<EvpModel<K, 1>>(d, points, npts);
+    case 3: return std::make_unique<EvpModel<K, 3>>(d, points, npts);
+    case 6: return std::make_unique<EvpModel<K, 6>>(d, points, npts);
+    default:
+      throw Error("elasto-viscoplastic model with " + std::to_string(d.nslip) +
+                  " slip systems is not built (supported: 1, 3, 6, 12)");
+  }
+}
+
+template std::unique_ptr<Model<float>> make_model<float>(const ModelDesc&, const std::vector<int>&,
+                                                         long long);
+template std::unique_ptr<Model<double>> make_model<double>(const ModelDesc&,
+                                                           const std::vector<int>&, long long);
+
+}  // namespace mtl

@@ -1,0 +1,269 @@
+// Device MINRES: the recurrence of SciPy's scipy/sparse/linalg/_isolve/minres.py
+// (Paige & Saunders 1975) for x0 = 0, no preconditioner, no shift -- the call
+// made at small_strain_fft.py:289-294 of the reference.  Scalars live in a
+// MinresState in device memory; each iteration is
+//   matvec (5 FFT kernels, epilogue u = Av - (beta/oldb) r1, partials <v,u>)
+//   k_alpha      alfa = <v,u>
+//   k_lanczos    y = u - (alfa/beta) r2 (in place of u), partials |y|^2
+//   k_givens     beta_{k+1}, plane rotation, w/x coefficients
+//   k_update     w = (v - oldeps w1 - delta w2)/gamma, x += phi w, partials |x|^2
+//   k_tests      SciPy's stopping tests -> done
+// and the host polls `done` once per chunk of iterations.
+#include <cfloat>
+#include <cmath>
+
+#include "minres.hpp"
+
+namespace mtl {
+
+namespace {
+
+constexpr int kVBT = 256;      // vector kernels
+constexpr int kVBlocks = 2048;  // grid-stride blocks for vector kernels
+constexpr double kEps = DBL_EPSILON;
+
+__device__ double sum_partials(const double* partials, int count, double* red) {
+  double acc = 0.0;
+  for (int i = threadIdx.x; i < count; i += blockDim.x) acc += partials[i];
+  return block_sum<kVBT>(acc, red);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kVBT) k_init(long long n6, const double* b, T* r2, T* x, T* w,
+                                                T* w2, double* partials) {
+  __shared__ double red[kVBT / 64];
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)kVBT + threadIdx.x; i < n6;
+       i += (long long)gridDim.x * kVBT) {
+    const T v = T(b[i]);
+    r2[i] = v;
+    x[i] = T(0);
+    w[i] = T(0);
+    w2[i] = T(0);
+    acc += double(v) * double(v);
+  }
+  const double t = block_sum<kVBT>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kVBT) k_start(MinresState* st, const double* partials, int np,
+                                                 double rtol, long long maxiter) {
+  __shared__ double red[kVBT / 64];
+  const double b2 = sum_partials(partials, np, red);
+  if (threadIdx.x != 0) return;
+  MinresState z{};
+  z.rtol = rtol;
+  z.maxiter = int(maxiter > 2147483647LL ? 2147483647LL : maxiter);
+  z.gmin = DBL_MAX;
+  z.cs = -1.0;
+  z.sn = 0.0;
+  if (b2 == 0.0) {  // beta1 == 0 (and |b| == 0): x = 0
+    z.done = 1;
+    *st = z;
+    return;
+  }
+  const double beta1 = sqrt(b2);
+  z.beta1 = beta1;
+  z.beta = beta1;
+  z.phibar = beta1;
+  z.rhs1 = beta1;
+  z.s = 1.0 / beta1;
+  z.c_r1 = 0.0;
+  *st = z;
+}
+
+__global__ void __launch_bounds__(kVBT) k_alpha(MinresState* st, const double* partials, int np) {
+  __shared__ double red[kVBT / 64];
+  if (st->done) return;
+  const double a = sum_partials(partials, np, red);
+  if (threadIdx.x != 0) return;
+  st->alfa = a;
+  st->c_r2 = a / st->beta;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kVBT) k_lanczos(long long n6, const MinresState* st, T* u,
+                                                   const T* r2, double* partials) {
+  __shared__ double red[kVBT / 64];
+  if (st->done) return;
+  const T c = T(st->c_r2);
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)kVBT + threadIdx.x; i < n6;
+       i += (long long)gridDim.x * kVBT) {
+    const T y = u[i] - c * r2[i];
+    u[i] = y;
+    acc += double(y) * double(y);
+  }
+  const double t = block_sum<kVBT>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kVBT) k_givens(MinresState* st, const double* partials, int np) {
+  __shared__ double red[kVBT / 64];
+  if (st->done) return;
+  const double b2 = sum_partials(partials, np, red);
+  if (threadIdx.x != 0) return;
+  MinresState z = *st;
+  z.itn += 1;
+  z.oldb = z.beta;
+  z.beta = sqrt(b2);
+  z.tnorm2 += z.alfa * z.alfa + z.oldb * z.oldb + z.beta * z.beta;
+  if (z.itn == 1 && z.beta / z.beta1 <= 10.0 * kEps) z.istop = -1;
+  z.oldeps = z.epsln;
+  z.delta = z.cs * z.dbar + z.sn * z.alfa;
+  const double gbar = z.sn * z.dbar - z.cs * z.alfa;
+  z.epsln = z.sn * z.beta;
+  z.dbar = -z.cs * z.beta;
+  z.root = sqrt(gbar * gbar + z.dbar * z.dbar);
+  const double gamma = fmax(sqrt(gbar * gbar + z.beta * z.beta), kEps);
+  z.cs = gbar / gamma;
+  z.sn = z.beta / gamma;
+  z.phi = z.cs * z.phibar;
+  z.phibar = z.sn * z.phibar;
+  z.denom = 1.0 / gamma;
+  z.gmax = fmax(z.gmax, gamma);
+  z.gmin = fmin(z.gmin, gamma);
+  const double zz = z.rhs1 / gamma;
+  z.rhs1 = z.rhs2 - z.delta * zz;
+  z.rhs2 = -z.epsln * zz;
+  z.anorm = sqrt(z.tnorm2);
+  // v_k = r2_old / beta_k feeds k_update; the next matvec uses 1/beta_{k+1}
+  z.s_prev = z.s;
+  z.s = 1.0 / z.beta;
+  z.c_r1 = z.beta / z.oldb;
+  z.pending_xnorm = 1;
+  *st = z;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kVBT) k_update(long long n6, const MinresState* st,
+                                                  const T* r2old, const T* w1, const T* w2,
+                                                  T* wnew, T* x, double* partials) {
+  __shared__ double red[kVBT / 64];
+  if (st->done) return;
+  const T sv = T(st->s_prev), oe = T(st->oldeps), de = T(st->delta), dn = T(st->denom),
+          ph = T(st->phi);
+  double acc = 0.0;
+  for (long long i = blockIdx.x * (long long)kVBT + threadIdx.x; i < n6;
+       i += (long long)gridDim.x * kVBT) {
+    const T v = sv * r2old[i];
+    const T w = (v - oe * w1[i] - de * w2[i]) * dn;
+    wnew[i] = w;
+    const T xn = x[i] + ph * w;
+    x[i] = xn;
+    acc += double(xn) * double(xn);
+  }
+  const double t = block_sum<kVBT>(acc, red);
+  if (threadIdx.x == 0) partials[blockIdx.x] = t;
+}
+
+__global__ void __launch_bounds__(kVBT) k_tests(MinresState* st, const double* partials, int np) {
+  __shared__ double red[kVBT / 64];
+  if (st->done) return;
+  const double x2 = sum_partials(partials, np, red);
+  if (threadIdx.x != 0) return;
+  MinresState z = *st;
+  z.pending_xnorm = 0;
+  z.ynorm = sqrt(x2);
+  const double epsx = z.anorm * z.ynorm * kEps;
+  z.rnorm = z.phibar;
+  z.test1 = (z.ynorm == 0.0 || z.anorm == 0.0) ? INFINITY : z.rnorm / (z.anorm * z.ynorm);
+  z.test2 = z.anorm == 0.0 ? INFINITY : z.root / z.anorm;
+  z.acond = z.gmax / z.gmin;
+  if (z.istop == 0) {
+    if (1.0 + z.test2 <= 1.0) z.istop = 2;
+    if (1.0 + z.test1 <= 1.0) z.istop = 1;
+    if (z.itn >= z.maxiter) z.istop = 6;
+    if (z.acond >= 0.1 / kEps) z.istop = 4;
+    if (epsx >= z.beta1) z.istop = 3;
+    if (z.test2 <= z.rtol) z.istop = 2;
+    if (z.test1 <= z.rtol) z.istop = 1;
+  }
+  if (z.istop != 0) z.done = 1;
+  *st = z;
+}
+
+int vec_blocks(long long n6) {
+  const long long b = (n6 + kVBT - 1) / kVBT;
+  return int(b < kVBlocks ? b : kVBlocks);
+}
+
+}  // namespace
+
+template <typename T>
+Minres<T>::Minres(Spectral<T>& sp) : sp_(sp) {
+  n6_ = 6 * sp.grid().n;
+  for (auto& v : vec_) MTL_HIP(hipMalloc(&v, n6_ * sizeof(T)));
+  MTL_HIP(hipMalloc(&x_, n6_ * sizeof(T)));
+  partial_cap_ = std::max(kVBlocks, sp.zinv_blocks());
+  MTL_HIP(hipMalloc(&partials_, partial_cap_ * sizeof(double)));
+  MTL_HIP(hipMalloc(&state_, sizeof(MinresState)));
+  MTL_HIP(hipHostMalloc(&host_state_, sizeof(MinresState), hipHostMallocDefault));
+}
+
+template <typename T>
+Minres<T>::~Minres() {
+  for (auto v : vec_)
+    if (v) (void)hipFree(v);
+  if (x_) (void)hipFree(x_);
+  if (partials_) (void)hipFree(partials_);
+  if (state_) (void)hipFree(state_);
+  if (host_state_) (void)hipHostFree(host_state_);
+}
+
+template <typename T>
+int Minres<T>::solve(const double* b, const T* tangent, const Vec6& mask, double rtol,
+                     long long maxiter, hipStream_t s, int* istop_out) {
+  // roles: r1, r2 (Lanczos), w1, w2 (previous w), wf (free slot for the new w)
+  T* r1 = vec_[0];
+  T* r2 = vec_[1];
+  T* w1 = vec_[2];
+  T* w2 = vec_[3];
+  T* wf = vec_[4];
+  const int vb = vec_blocks(n6_);
+  hipLaunchKernelGGL(k_init<T>, dim3(vb), dim3(kVBT), 0, s, n6_, b, r2, x_, w2, wf, partials_);
+  MTL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_start, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb, rtol, maxiter);
+  MTL_LAUNCH_CHECK();
+  // at itn 1 the previous w's are zero: w1 := wf (zeroed), w2 (zeroed)
+  std::swap(w1, wf);
+
+  int chunk = 8;
+  long long launched = 0;
+  while (true) {
+    for (int it = 0; it < chunk; ++it) {
+      const int np = sp_.matvec_minres(r2, tangent, r1, mask, state_, partials_, s);
+      hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kVBT), 0, s, state_, partials_, np);
+      MTL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_lanczos<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, r2, partials_);
+      MTL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_givens, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb);
+      MTL_LAUNCH_CHECK();
+      // Lanczos roles rotate: r1 <- r2 (old), r2 <- y (stored in old r1)
+      std::swap(r1, r2);
+      // v_k = r2_old / beta_k now lives in r1
+      hipLaunchKernelGGL(k_update<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, w1, w2, wf, x_,
+                         partials_);
+      MTL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_tests, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb);
+      MTL_LAUNCH_CHECK();
+      // w roles rotate: w1 <- w2, w2 <- new w, free slot <- old w1
+      T* old_w1 = w1;
+      w1 = w2;
+      w2 = wf;
+      wf = old_w1;
+      ++launched;
+    }
+    MTL_HIP(hipMemcpyAsync(host_state_, state_, sizeof(MinresState), hipMemcpyDeviceToHost, s));
+    MTL_HIP(hipStreamSynchronize(s));
+    if (host_state_->done || launched >= maxiter) break;
+    if (chunk < 64) chunk *= 2;
+  }
+  if (istop_out) *istop_out = host_state_->istop;
+  return host_state_->itn;
+}
+
+template class Minres<float>;
+template class Minres<double>;
+
+}  // namespace mtl

@@ -1,0 +1,662 @@
+// Newton-Krylov load increment of SmallStrainFFT on the device, and the C ABI.
+//
+// Restates small_strain_fft.py:106-246 of the reference: predictor, Newton loop
+// (constitutive update, residual b = Ghat*(Sigma - sigma), MINRES correction,
+// Abaqus-style equilibrium / strain-correction tests), commit and cutback.
+// Host code keeps only scalars; every field lives in HBM as SoA [6][N] FP64
+// (strain, stress, residual) or [21][N] K (tangent, K = float or double).
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "../../include/materialite_amd.h"
+#include "constitutive.hpp"
+#include "minres.hpp"
+
+namespace mtl {
+
+bool inverse6_host(const double* A, double* out);
+
+namespace {
+
+constexpr int kNBT = 256;
+constexpr int kNormBlocks = 1024;
+constexpr int kNormWidth = 9;  // max|b|, sum sigma (6), max|dev dsigma|^2, max|D:dx|^2
+
+__global__ void __launch_bounds__(kNBT) k_predictor(long long n, const double* eps_old,
+                                                     double* fluct, double* eps, Vec6 de, Vec6 pred,
+                                                     int reset) {
+  for (long long p = blockIdx.x * (long long)kNBT + threadIdx.x; p < n;
+       p += (long long)gridDim.x * kNBT) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      double f = fluct[c * n + p];
+      if (reset) {
+        f = 0.0;
+        fluct[c * n + p] = 0.0;
+      }
+      eps[c * n + p] = eps_old[c * n + p] + de.v[c] + f + pred.v[c];
+    }
+  }
+}
+
+template <typename K>
+__global__ void __launch_bounds__(kNBT) k_guess(long long n, int step0, const double* sig_old,
+                                                 const K* D, Vec6 ds, Vec6 de, double* guess) {
+  for (long long p = blockIdx.x * (long long)kNBT + threadIdx.x; p < n;
+       p += (long long)gridDim.x * kNBT) {
+#pragma unroll
+    for (int a = 0; a < 6; ++a) {
+      double g;
+      if (step0) {
+        double acc = 0.0;
+#pragma unroll
+        for (int b = 0; b < 6; ++b) acc += double(D[sym6(a, b) * n + p]) * de.v[b];
+        g = ds.v[a] + acc;
+      } else {
+        g = sig_old[a * n + p] + ds.v[a];
+      }
+      guess[a * n + p] = g;
+    }
+  }
+}
+
+template <typename K>
+__global__ void __launch_bounds__(kNBT) k_add_correction(long long n6, const K* x, double* eps,
+                                                          double* fluct) {
+  for (long long i = blockIdx.x * (long long)kNBT + threadIdx.x; i < n6;
+       i += (long long)gridDim.x * kNBT) {
+    const double d = double(x[i]);
+    eps[i] += d;
+    fluct[i] += d;
+  }
+}
+
+template <typename K>
+__global__ void __launch_bounds__(kNBT) k_newton_norms(long long n, const double* b,
+                                                        const double* sig, const double* sig_old,
+                                                        const K* D, const K* x, double* partials) {
+  __shared__ double red[kNBT / 64];
+  double mb = 0.0, ms = 0.0, mdx = 0.0;
+  double ss[6] = {0, 0, 0, 0, 0, 0};
+  for (long long p = blockIdx.x * (long long)kNBT + threadIdx.x; p < n;
+       p += (long long)gridDim.x * kNBT) {
+    double ds[6], xv[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      mb = fmax(mb, fabs(b[c * n + p]));
+      const double s = sig[c * n + p];
+      ss[c] += s;
+      ds[c] = s - sig_old[c * n + p];
+      xv[c] = x ? double(x[c * n + p]) : 0.0;
+    }
+    const double tr = (ds[0] + ds[1] + ds[2]) / 3.0;
+    double d2 = 0.0;
+#pragma unroll
+    for (int c = 0; c < 6; ++c) {
+      const double v = c < 3 ? ds[c] - tr : ds[c];
+      d2 += v * v;
+    }
+    ms = fmax(ms, d2);
+    if (x) {
+      double q2 = 0.0;
+#pragma unroll
+      for (int a = 0; a < 6; ++a) {
+        double acc = 0.0;
+#pragma unroll
+        for (int c = 0; c < 6; ++c) acc += double(D[sym6(a, c) * n + p]) * xv[c];
+        q2 += acc * acc;
+      }
+      mdx = fmax(mdx, q2);
+    }
+  }
+  double out[kNormWidth];
+  out[0] = block_max<kNBT>(mb, red);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) out[1 + c] = block_sum<kNBT>(ss[c], red);
+  out[7] = block_max<kNBT>(ms, red);
+  out[8] = block_max<kNBT>(mdx, red);
+  if (threadIdx.x == 0)
+#pragma unroll
+    for (int k = 0; k < kNormWidth; ++k) partials[blockIdx.x * kNormWidth + k] = out[k];
+}
+
+template <typename K>
+__global__ void __launch_bounds__(kNBT) k_tangent_sum(long long n, const K* D, double* partials) {
+  __shared__ double red[kNBT / 64];
+  double acc[21];
+#pragma unroll
+  for (int k = 0; k < 21; ++k) acc[k] = 0.0;
+  for (long long p = blockIdx.x * (long long)kNBT + threadIdx.x; p < n;
+       p += (long long)gridDim.x * kNBT)
+#pragma unroll
+    for (int k = 0; k < 21; ++k) acc[k] += double(D[k * n + p]);
+#pragma unroll
+  for (int k = 0; k < 21; ++k) {
+    const double t = block_sum<kNBT>(acc[k], red);
+    if (threadIdx.x == 0) partials[blockIdx.x * 21 + k] = t;
+  }
+}
+
+__global__ void k_scale(long long count, double* a, double f) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x)
+    a[i] *= f;
+}
+
+template <typename From, typename To>
+__global__ void k_convert(long long count, const From* a, To* b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x)
+    b[i] = To(a[i]);
+}
+
+int grid_for(long long count) {
+  return int(std::max<long long>(1, std::min<long long>((count + kNBT - 1) / kNBT, kNormBlocks)));
+}
+
+thread_local std::string g_last_error;
+
+}  // namespace
+
+struct SolverBase {
+  virtual ~SolverBase() = default;
+  virtual int add_model(const ModelDesc& d) = 0;
+  virtual void set_microstructure(const double* rot, const int32_t* mop) = 0;
+  virtual void initialize() = 0;
+  virtual void step(const mtl_step_params& p, mtl_step_report& r) = 0;
+  virtual void commit(double scale) = 0;
+  virtual void rollback() = 0;
+  virtual void get_field(int which, double* out, int* ncomp) = 0;
+  virtual void project(const double* in, double* out, const double mask[6]) = 0;
+  virtual int minres(const double* b, double* x, const double mask[6], double rtol) = 0;
+  virtual int constitutive(const double* eps, const double* guess, double dt, double* sig) = 0;
+};
+
+template <typename K>
+class Solver final : public SolverBase {
+ public:
+  Solver(int nx, int ny, int nz) : sp64_(nx, ny, nz) {
+    if constexpr (std::is_same<K, double>::value) {
+      spk_ = &sp64_;
+    } else {
+      own_spk_ = std::make_unique<Spectral<K>>(nx, ny, nz);
+      spk_ = own_spk_.get();
+    }
+    minres_ = std::make_unique<Minres<K>>(*spk_);
+    n_ = sp64_.grid().n;
+    MTL_HIP(hipStreamCreate(&s_));
+    auto dalloc = [&](double*& p, long long c) { MTL_HIP(hipMalloc(&p, c * sizeof(double))); };
+    for (double** f : {&eps_, &sig_, &sig_old_, &eps_old_, &fluct_, &guess_, &b_}) dalloc(*f, 6 * n_);
+    dalloc(rot_, 9 * n_);
+    MTL_HIP(hipMalloc(&D_, 21 * n_ * sizeof(K)));
+    MTL_HIP(hipMalloc(&D_old_, 21 * n_ * sizeof(K)));
+    dalloc(partials_, kNormBlocks * 21);
+    MTL_HIP(hipHostMalloc(&hpart_, kNormBlocks * 21 * sizeof(double), hipHostMallocDefault));
+  }
+  ~Solver() override {
+    models_.clear();
+    for (double* p : {eps_, sig_, sig_old_, eps_old_, fluct_, guess_, b_, rot_, partials_})
+      if (p) (void)hipFree(p);
+    if (D_) (void)hipFree(D_);
+    if (D_old_) (void)hipFree(D_old_);
+    if (hpart_) (void)hipHostFree(hpart_);
+    if (s_) (void)hipStreamDestroy(s_);
+  }
+
+  int add_model(const ModelDesc& d) override {
+    if (d.kind == 1 && (d.nslip < 1 || int(d.schmid.size()) != 6 * d.nslip))
+      throw Error("bad slip-system table");
+    descs_.push_back(d);
+    return int(descs_.size()) - 1;
+  }
+
+  void set_microstructure(const double* rot, const int32_t* mop) override {
+    std::vector<double> soa(9 * size_t(n_));
+    for (long long p = 0; p < n_; ++p)
+      for (int k = 0; k < 9; ++k) soa[size_t(k) * n_ + p] = rot[p * 9 + k];
+    MTL_HIP(hipMemcpy(rot_, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+    std::vector<std::vector<int>> pts(descs_.size());
+    for (long long p = 0; p < n_; ++p) {
+      const int m = mop ? mop[p] : 0;
+      if (m < 0 || m >= int(descs_.size())) throw Error("model id out of range at point " + std::to_string(p));
+      pts[m].push_back(int(p));
+    }
+    models_.clear();
+    for (size_t m = 0; m < descs_.size(); ++m) models_.push_back(make_model<K>(descs_[m], pts[m], n_));
+  }
+
+  void initialize() override {
+    if (models_.empty()) throw Error("set the microstructure before initialize");
+    for (double* f : {eps_, sig_, sig_old_, eps_old_, fluct_, guess_, b_})
+      MTL_HIP(hipMemsetAsync(f, 0, 6 * n_ * sizeof(double), s_));
+    MTL_HIP(hipMemsetAsync(D_, 0, 21 * n_ * sizeof(K), s_));
+    for (auto& m : models_) m->init_tangent(rot_, D_, s_);
+    MTL_HIP(hipMemcpyAsync(D_old_, D_, 21 * n_ * sizeof(K), hipMemcpyDeviceToDevice, s_));
+    MTL_HIP(hipStreamSynchronize(s_));
+  }
+
+  // Multiphase dispatch (multiphase.py:38-51): fails as soon as a model fails
+  int constitutive_all(const double* eps, const double* guess, double dt, double* sig) {
+    int iters = 0;
+    for (auto& m : models_) {
+      const int it = m->calculate(rot_, eps, guess, dt, sig, D_, s_);
+      if (!it) return 0;
+      iters = std::max(iters, it);
+    }
+    return iters;
+  }
+
+  struct Norms {
+    double maxb, mean[6], maxdev2, maxdx2;
+  };
+
+  Norms norms(bool with_dx) {
+    const int nb = grid_for(n_);
+    hipLaunchKernelGGL(k_newton_norms<K>, dim3(nb), dim3(kNBT), 0, s_, n_, b_, sig_, sig_old_, D_,
+                       with_dx ? minres_->x() : nullptr, partials_);
+    MTL_LAUNCH_CHECK();
+    MTL_HIP(hipMemcpyAsync(hpart_, partials_, nb * kNormWidth * sizeof(double),
+                           hipMemcpyDeviceToHost, s_));
+    MTL_HIP(hipStreamSynchronize(s_));
+    Norms r{};
+    for (int i = 0; i < nb; ++i) {
+      const double* q = hpart_ + i * kNormWidth;
+      r.maxb = std::max(r.maxb, q[0]);
+      for (int c = 0; c < 6; ++c) r.mean[c] += q[1 + c];
+      r.maxdev2 = std::max(r.maxdev2, q[7]);
+      r.maxdx2 = std::max(r.maxdx2, q[8]);
+    }
+    for (double& m : r.mean) m /= double(n_);
+    return r;
+  }
+
+  static double von_mises_of_mean(const double* m) {
+    // stress.mean().dev.norm * sqrt(1.5)  (small_strain_fft.py:141-143)
+    const double tr = (m[0] + m[1] + m[2]) / 3.0;
+    double s = 0.0;
+    for (int c = 0; c < 6; ++c) {
+      const double v = c < 3 ? m[c] - tr : m[c];
+      s += v * v;
+    }
+    return std::sqrt(s) * std::sqrt(1.5);
+  }
+
+  void tangent_mean(double out36[36]) {
+    const int nb = grid_for(n_);
+    hipLaunchKernelGGL(k_tangent_sum<K>, dim3(nb), dim3(kNBT), 0, s_, n_, D_, partials_);
+    MTL_LAUNCH_CHECK();
+    MTL_HIP(hipMemcpyAsync(hpart_, partials_, nb * 21 * sizeof(double), hipMemcpyDeviceToHost, s_));
+    MTL_HIP(hipStreamSynchronize(s_));
+    double sum[21] = {};
+    for (int i = 0; i < nb; ++i)
+      for (int k = 0; k < 21; ++k) sum[k] += hpart_[i * 21 + k];
+    for (int a = 0; a < 6; ++a)
+      for (int b = 0; b < 6; ++b) out36[a * 6 + b] = sum[sym6(a, b)] / double(n_);
+  }
+
+  void residual(const Vec6& applied, const Vec6& mask) {
+    // b = Ghat * (Sigma_applied - sigma), always in FP64
+    sp64_.project(sig_, b_, applied, -1.0, mask, s_);
+  }
+
+  void step(const mtl_step_params& P, mtl_step_report& R) override {
+    std::memset(&R, 0, sizeof(R));
+    Vec6 de, ds, applied, mask, pred{};
+    std::memcpy(de.v, P.strain_increment, sizeof(de.v));
+    std::memcpy(ds.v, P.stress_increment, sizeof(ds.v));
+    std::memcpy(applied.v, P.applied_stress, sizeof(applied.v));
+    std::memcpy(mask.v, P.stress_mask, sizeof(mask.v));
+    const double dt = P.time_increment;
+    // predictor (small_strain_fft.py:117-123)
+    bool all_small = true;
+    for (double v : ds.v) all_small = all_small && (v < 1.0e-14);
+    if (!all_small) {
+      double dm[36], dinv[36];
+      tangent_mean(dm);
+      if (!inverse6_host(dm, dinv)) throw Error("mean tangent is singular");
+      for (int a = 0; a < 6; ++a) {
+        double acc = 0.0;
+        for (int b = 0; b < 6; ++b) acc += dinv[a * 6 + b] * ds.v[b];
+        pred.v[a] = acc;
+      }
+    }
+    const int nb = grid_for(n_);
+    hipLaunchKernelGGL(k_predictor, dim3(nb), dim3(kNBT), 0, s_, n_, eps_old_, fluct_, eps_, de,
+                       pred, P.reset_fluctuation);
+    MTL_LAUNCH_CHECK();
+    hipLaunchKernelGGL(k_guess<K>, dim3(nb), dim3(kNBT), 0, s_, n_, P.time_step_id == 0 ? 1 : 0,
+                       sig_old_, D_, ds, de, guess_);
+    MTL_LAUNCH_CHECK();
+
+    const int kMaxNewton = 100;  // small_strain_fft.py:125
+    const long long maxiter = 5LL * 6LL * n_;
+    bool converged = false;
+    int iteration = 0;
+    int max_constit = 0;
+    double eq_err = 0.0, vm = 0.0, old_eq = 0.0;
+    while (!converged) {
+      ++iteration;
+      if (iteration == 1) {
+        const int it = constitutive_all(eps_, guess_, dt, sig_);
+        if (!it) {
+          restore_tangent();
+          R.constitutive_failed = 1;
+          break;
+        }
+        residual(applied, mask);
+        const Norms nm = norms(false);
+        vm = von_mises_of_mean(nm.mean);
+        eq_err = nm.maxb / vm;
+        copy6(guess_, sig_);
+        max_constit = std::max(max_constit, it);
+      }
+      old_eq = std::max(eq_err, P.global_tolerance);
+      int istop = 0;
+      const int itn = minres_->solve(b_, D_, mask, P.linear_solver_tolerance, maxiter, s_, &istop);
+      R.krylov_iterations += itn;
+      hipLaunchKernelGGL(k_add_correction<K>, dim3(nb), dim3(kNBT), 0, s_, 6 * n_, minres_->x(), eps_,
+                         fluct_);
+      MTL_LAUNCH_CHECK();
+      const int it = constitutive_all(eps_, guess_, dt, sig_);
+      if (!it) {
+        restore_tangent();
+        R.constitutive_failed = 1;
+        break;
+      }
+      max_constit = std::max(max_constit, it);
+      residual(applied, mask);
+      const Norms nm = norms(true);
+      vm = von_mises_of_mean(nm.mean);
+      const double time_avg = (P.summed_von_mises_stress + vm) / double(P.time_step_id + 1);
+      if (vm / time_avg < 1.0e-12) vm = time_avg;
+      eq_err = nm.maxb / vm;
+      double max_ds = std::sqrt(nm.maxdev2) * std::sqrt(1.5);
+      if (max_ds / vm < 1.0e-12) max_ds = vm;
+      const double strain_err = std::sqrt(nm.maxdx2) / max_ds;
+      const double projected = strain_err * eq_err / old_eq;
+      if (eq_err < P.global_tolerance &&
+          (strain_err < P.strain_correction_tolerance ||
+           projected < P.strain_correction_tolerance || P.time_step_id == 0 || eq_err < 1e-8))
+        converged = true;
+      copy6(guess_, sig_);
+      R.scaled_strain_error = strain_err;
+      R.projected_scaled_strain_error = projected;
+      if (iteration == kMaxNewton && !converged) break;
+    }
+    R.converged = converged ? 1 : 0;
+    R.newton_iterations = iteration;
+    R.max_constitutive_iterations = max_constit;
+    R.mean_von_mises_stress = vm;
+    R.equilibrium_error = eq_err;
+  }
+
+  void commit(double scale) override {
+    for (auto& m : models_) m->commit(rot_, s_);
+    copy6(sig_old_, sig_);
+    copy6(eps_old_, eps_);
+    MTL_HIP(hipMemcpyAsync(D_old_, D_, 21 * n_ * sizeof(K), hipMemcpyDeviceToDevice, s_));
+    scale6(fluct_, scale);
+    MTL_HIP(hipStreamSynchronize(s_));
+  }
+
+  void rollback() override {
+    scale6(fluct_, 0.5);
+    MTL_HIP(hipStreamSynchronize(s_));
+  }
+
+  void get_field(int which, double* out, int* ncomp) override {
+    if (which == kFieldStress || which == kFieldStrain) {
+      soa_to_host(which == kFieldStress ? sig_ : eps_, 6, out);
+      *ncomp = 6;
+      return;
+    }
+    if (which == kFieldTangent) {
+      std::vector<K> soa(21 * size_t(n_));
+      MTL_HIP(hipMemcpy(soa.data(), D_, soa.size() * sizeof(K), hipMemcpyDeviceToHost));
+      for (long long p = 0; p < n_; ++p)
+        for (int a = 0; a < 6; ++a)
+          for (int b = 0; b < 6; ++b) out[p * 36 + a * 6 + b] = double(soa[size_t(sym6(a, b)) * n_ + p]);
+      *ncomp = 36;
+      return;
+    }
+    // state fields: scatter each model's rows; points of models without the
+    // field are zero (multiphase.py:75-95)
+    int width = 0;
+    std::vector<std::pair<const Model<K>*, std::vector<double>>> parts;
+    for (auto& m : models_) {
+      std::vector<double> v;
+      int nc = 0;
+      if (m->field(which, rot_, v, nc, s_)) {
+        width = std::max(width, nc);
+        parts.emplace_back(m.get(), std::move(v));
+      }
+    }
+    if (width == 0) throw Error("field not available in any constitutive model");
+    std::fill(out, out + size_t(width) * n_, 0.0);
+    for (auto& pr : parts) {
+      const auto& pts = pr.first->points();
+      const size_t nc = pr.second.size() / std::max<size_t>(pts.size(), 1);
+      for (size_t i = 0; i < pts.size(); ++i)
+        for (size_t c = 0; c < nc; ++c) out[size_t(pts[i]) * width + c] = pr.second[i * nc + c];
+    }
+    *ncomp = width;
+  }
+
+  void project(const double* in, double* out, const double m[6]) override {
+    Vec6 mask, zero{};
+    std::memcpy(mask.v, m, sizeof(mask.v));
+    spk_->project(in, out, zero, 1.0, mask, s_);
+    MTL_HIP(hipStreamSynchronize(s_));
+  }
+
+  int minres(const double* b, double* x, const double m[6], double rtol) override {
+    Vec6 mask;
+    std::memcpy(mask.v, m, sizeof(mask.v));
+    int istop = 0;
+    const int itn = minres_->solve(b, D_, mask, rtol, 5LL * 6LL * n_, s_, &istop);
+    hipLaunchKernelGGL((k_convert<K, double>), dim3(grid_for(6 * n_)), dim3(kNBT), 0, s_, 6 * n_,
+                       minres_->x(), x);
+    MTL_LAUNCH_CHECK();
+    MTL_HIP(hipStreamSynchronize(s_));
+    return itn;
+  }
+
+  int constitutive(const double* eps, const double* guess, double dt, double* sig) override {
+    const int it = constitutive_all(eps, guess, dt, sig);
+    MTL_HIP(hipStreamSynchronize(s_));
+    return it;
+  }
+
+ private:
+  void copy6(double* dst, const double* src) {
+    MTL_HIP(hipMemcpyAsync(dst, src, 6 * n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
+  }
+  void scale6(double* a, double f) {
+    hipLaunchKernelGGL(k_scale, dim3(grid_for(6 * n_)), dim3(kNBT), 0, s_, 6 * n_, a, f);
+    MTL_LAUNCH_CHECK();
+  }
+  void restore_tangent() {
+    MTL_HIP(hipMemcpyAsync(D_, D_old_, 21 * n_ * sizeof(K), hipMemcpyDeviceToDevice, s_));
+  }
+  void soa_to_host(const double* d, int nc, double* out) {
+    std::vector<double> soa(size_t(nc) * n_);
+    MTL_HIP(hipMemcpyAsync(soa.data(), d, soa.size() * sizeof(double), hipMemcpyDeviceToHost, s_));
+    MTL_HIP(hipStreamSynchronize(s_));
+    for (long long p = 0; p < n_; ++p)
+      for (int c = 0; c < nc; ++c) out[p * nc + c] = soa[size_t(c) * n_ + p];
+  }
+
+  Spectral<double> sp64_;
+  std::unique_ptr<Spectral<K>> own_spk_;
+  Spectral<K>* spk_ = nullptr;
+  std::unique_ptr<Minres<K>> minres_;
+  long long n_ = 0;
+  hipStream_t s_ = nullptr;
+  double *eps_ = nullptr, *sig_ = nullptr, *sig_old_ = nullptr, *eps_old_ = nullptr,
+         *fluct_ = nullptr, *guess_ = nullptr, *b_ = nullptr, *rot_ = nullptr;
+  K *D_ = nullptr, *D_old_ = nullptr;
+  double* partials_ = nullptr;
+  double* hpart_ = nullptr;
+  std::vector<ModelDesc> descs_;
+  std::vector<std::unique_ptr<Model<K>>> models_;
+};
+
+bool inverse6_host(const double* A, double* out) {
+  double m[6][12];
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 12; ++j) m[i][j] = j < 6 ? A[i * 6 + j] : (j - 6 == i ? 1.0 : 0.0);
+  for (int c = 0; c < 6; ++c) {
+    int piv = c;
+    for (int r = c + 1; r < 6; ++r)
+      if (std::fabs(m[r][c]) > std::fabs(m[piv][c])) piv = r;
+    if (m[piv][c] == 0.0) return false;
+    if (piv != c)
+      for (int j = 0; j < 12; ++j) std::swap(m[c][j], m[piv][j]);
+    const double inv = 1.0 / m[c][c];
+    for (int j = 0; j < 12; ++j) m[c][j] *= inv;
+    for (int r = 0; r < 6; ++r)
+      if (r != c) {
+        const double f = m[r][c];
+        if (f != 0.0)
+          for (int j = 0; j < 12; ++j) m[r][j] -= f * m[c][j];
+      }
+  }
+  for (int i = 0; i < 6; ++i)
+    for (int j = 0; j < 6; ++j) out[i * 6 + j] = m[i][j + 6];
+  return true;
+}
+
+}  // namespace mtl
+
+// ====================================================================== C ABI
+struct mtl_solver {
+  std::unique_ptr<mtl::SolverBase> impl;
+};
+
+namespace {
+template <typename F>
+int guarded(F&& f) {
+  try {
+    f();
+    return 0;
+  } catch (const std::exception& e) {
+    mtl::g_last_error = e.what();
+    return -1;
+  } catch (...) {
+    mtl::g_last_error = "unknown error";
+    return -1;
+  }
+}
+int bad(const char* msg) {
+  mtl::g_last_error = msg;
+  return -2;
+}
+}  // namespace
+
+extern "C" {
+
+const char* mtl_last_error(void) { return mtl::g_last_error.c_str(); }
+
+int mtl_solver_create(int nx, int ny, int nz, int precision, mtl_solver** out) {
+  if (!out) return bad("null output handle");
+  if (precision != MTL_FP64 && precision != MTL_MIXED) return bad("unknown precision");
+  return guarded([&] {
+    auto h = std::make_unique<mtl_solver>();
+    if (precision == MTL_FP64)
+      h->impl = std::make_unique<mtl::Solver<double>>(nx, ny, nz);
+    else
+      h->impl = std::make_unique<mtl::Solver<float>>(nx, ny, nz);
+    *out = h.release();
+  });
+}
+
+int mtl_solver_destroy(mtl_solver* h) {
+  return guarded([&] { delete h; });
+}
+
+int mtl_solver_add_elastic(mtl_solver* h, const double stiffness[36], int* model_id) {
+  if (!h || !stiffness || !model_id) return bad("null argument");
+  return guarded([&] {
+    mtl::ModelDesc d;
+    d.kind = 0;
+    std::memcpy(d.C, stiffness, sizeof(d.C));
+    *model_id = h->impl->add_model(d);
+  });
+}
+
+int mtl_solver_add_evp(mtl_solver* h, const double stiffness[36], int num_slip,
+                       const double* schmid, double reference_slip_rate, double rate_exponent,
+                       double slip_resistance, int hardening, const double hardening_props[4],
+                       int* model_id) {
+  if (!h || !stiffness || !schmid || !model_id) return bad("null argument");
+  if (hardening < 0 || hardening > 3) return bad("unknown hardening law");
+  return guarded([&] {
+    mtl::ModelDesc d;
+    d.kind = 1;
+    std::memcpy(d.C, stiffness, sizeof(d.C));
+    d.nslip = num_slip;
+    d.schmid.assign(schmid, schmid + 6 * num_slip);
+    d.gamma0 = reference_slip_rate;
+    d.n = rate_exponent;
+    d.g0 = slip_resistance;
+    d.hardening = hardening;
+    if (hardening_props) std::memcpy(d.hp, hardening_props, sizeof(d.hp));
+    *model_id = h->impl->add_model(d);
+  });
+}
+
+int mtl_solver_set_microstructure(mtl_solver* h, const double* rotations,
+                                  const int32_t* model_of_point) {
+  if (!h || !rotations) return bad("null argument");
+  return guarded([&] { h->impl->set_microstructure(rotations, model_of_point); });
+}
+
+int mtl_solver_initialize(mtl_solver* h) {
+  if (!h) return bad("null handle");
+  return guarded([&] { h->impl->initialize(); });
+}
+
+int mtl_solver_step(mtl_solver* h, const mtl_step_params* p, mtl_step_report* r) {
+  if (!h || !p || !r) return bad("null argument");
+  return guarded([&] { h->impl->step(*p, *r); });
+}
+
+int mtl_solver_commit(mtl_solver* h, double fluctuation_scale) {
+  if (!h) return bad("null handle");
+  return guarded([&] { h->impl->commit(fluctuation_scale); });
+}
+
+int mtl_solver_rollback(mtl_solver* h) {
+  if (!h) return bad("null handle");
+  return guarded([&] { h->impl->rollback(); });
+}
+
+int mtl_solver_get_field(mtl_solver* h, int field, double* out, int* ncomp) {
+  if (!h || !out || !ncomp) return bad("null argument");
+  return guarded([&] { h->impl->get_field(field, out, ncomp); });
+}
+
+int mtl_solver_project(mtl_solver* h, const double* in_dev, double* out_dev,
+                       const double stress_mask[6]) {
+  if (!h || !in_dev || !out_dev || !stress_mask) return bad("null argument");
+  return guarded([&] { h->impl->project(in_dev, out_dev, stress_mask); });
+}
+
+int mtl_solver_minres(mtl_solver* h, const double* b_dev, double* x_dev,
+                      const double stress_mask[6], double rtol, int* iterations) {
+  if (!h || !b_dev || !x_dev || !stress_mask || !iterations) return bad("null argument");
+  return guarded([&] { *iterations = h->impl->minres(b_dev, x_dev, stress_mask, rtol); });
+}
+
+int mtl_solver_constitutive(mtl_solver* h, const double* strain_dev, const double* guess_dev,
+                            double time_increment, double* stress_dev, int* iterations) {
+  if (!h || !strain_dev || !guess_dev || !stress_dev || !iterations) return bad("null argument");
+  return guarded(
+      [&] { *iterations = h->impl->constitutive(strain_dev, guess_dev, time_increment, stress_dev); });
+}
+
+}  // extern "C"

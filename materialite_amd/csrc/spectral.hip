@@ -1,0 +1,442 @@
+// FFT passes of the projection apply and of the MINRES matvec (gfx950).
+//
+// Data layout (all SoA): real fields [6][n]; tangent [21][n] packed upper
+// triangle; spectrum [6][nx][ny][nzc] complex.  One matvec is five kernels:
+//   z_forward  : D:(s r2) (or a + sgn*in), paired real->complex z-FFT
+//   y_pass     : complex y-FFT of kz tiles (forward)
+//   x_ghat     : complex x-FFT, Ghat applied per wave vector, inverse x-FFT
+//   y_pass     : inverse y-FFT
+//   z_inverse  : complex->real z-FFT, 1/n scaling, MINRES epilogue or store
+// Ghat:X = a(x)n + n(x)a - (n.a) n(x)n with a = X.n, n = t/|t|,
+// t_j = tan(pi q_j) (rotated finite-difference frequencies of
+// small_strain_fft.py:301-328); zero on even-N Nyquist planes (:354-359) and
+// diag(mask) at k = 0 (:363-367).
+#include <cmath>
+#include <vector>
+
+#include "spectral.hpp"
+
+namespace mtl {
+
+namespace {
+
+constexpr int kZBT = 256, kZPTS = 16;  // z passes: <= 4096 points per workgroup
+constexpr int kYBT = 256, kYPTS = 16;  // y pass:   <= 4096 points per workgroup
+constexpr int kXBT = 512, kXPTS = 16;  // x pass:   <= 8192 points per workgroup
+
+enum { kModeProject = 0, kModeMatvec = 1 };
+
+template <typename T>
+__device__ __forceinline__ void load_twiddles(cplx<T>* dst, const cplx<T>* src, int L) {
+  for (int i = threadIdx.x; i < L; i += blockDim.x) dst[i] = src[i];
+}
+
+// ----------------------------------------------------------------- z forward
+template <typename T, int MODE, typename TIn>
+__global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, AxisPlan pz, const cplx<T>* twz,
+                                                     int P, const TIn* in, const T* tangent,
+                                                     Vec6 a, double sgn,
+                                                     const MinresState* st, cplx<T>* spec) {
+  if (MODE == kModeMatvec && st->done) return;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + g.nz;
+  const int nz = g.nz, nzc = g.nzc;
+  const long long nlines = (long long)g.nx * g.ny;
+  const long long line0 = (long long)blockIdx.x * P;
+  load_twiddles(tw, twz, nz);
+
+  T s = T(1);
+  if (MODE == kModeMatvec) s = T(st->s);
+  for (int i = threadIdx.x; i < P * nz; i += kZBT) {
+    const int li = i / nz, z = i - li * nz;
+    const long long gl = line0 + li;
+    T tau[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
+    if (gl < nlines) {
+      const long long p = gl * nz + z;
+      if (MODE == kModeProject) {
+#pragma unroll
+        for (int c = 0; c < 6; ++c) tau[c] = T(a.v[c] + sgn * double(in[c * g.n + p]));
+      } else {
+        T v[6];
+#pragma unroll
+        for (int c = 0; c < 6; ++c) v[c] = s * T(in[c * g.n + p]);
+#pragma unroll
+        for (int i2 = 0; i2 < 6; ++i2) {
+          T acc = T(0);
+#pragma unroll
+          for (int j = 0; j < 6; ++j) acc += tangent[sym6(i2, j) * g.n + p] * v[j];
+          tau[i2] = acc;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 3; ++q) buf[(li * 3 + q) * nz + z] = {tau[2 * q], tau[2 * q + 1]};
+  }
+  __syncthreads();
+  block_fft<T, kZBT, kZPTS>(buf, 3 * P, pz, tw, false);
+
+  const T half = T(0.5);
+  for (int i = threadIdx.x; i < P * 3 * nzc; i += kZBT) {
+    const int li = i / (3 * nzc);
+    const int rem = i - li * 3 * nzc;
+    const int q = rem / nzc, kz = rem - q * nzc;
+    const long long gl = line0 + li;
+    if (gl >= nlines) continue;
+    const cplx<T>* line = buf + (li * 3 + q) * nz;
+    const cplx<T> zf = line[kz];
+    const cplx<T> zr = cconj(line[(nz - kz) % nz]);
+    const cplx<T> xa = {half * (zf.x + zr.x), half * (zf.y + zr.y)};
+    const cplx<T> xb = {half * (zf.y - zr.y), -half * (zf.x - zr.x)};
+    spec[(2 * q) * g.nspec + gl * nzc + kz] = xa;
+    spec[(2 * q + 1) * g.nspec + gl * nzc + kz] = xb;
+  }
+}
+
+// ----------------------------------------------------------------- y pass
+template <typename T>
+__global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, AxisPlan py, const cplx<T>* twy, int TY,
+                                                  int ntiles, bool inv,
+                                                  const MinresState* st, cplx<T>* spec) {
+  if (st != nullptr && st->done) return;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + g.ny;
+  const int b = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int tile = b % ntiles;
+  const int cx = b / ntiles;  // c * nx + x
+  const int kz0 = tile * TY;
+  const int width = min(TY, g.nzc - kz0);
+  const int ny = g.ny;
+  cplx<T>* base = spec + (long long)cx * ny * g.nzc + kz0;  // spec[c][x][.][kz0]
+  load_twiddles(tw, twy, ny);
+  for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
+    const int y = i / TY, t = i - y * TY;
+    buf[t * ny + y] = t < width ? base[(long long)y * g.nzc + t] : cplx<T>{T(0), T(0)};
+  }
+  __syncthreads();
+  block_fft<T, kYBT, kYPTS>(buf, TY, py, tw, inv);
+  for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
+    const int y = i / TY, t = i - y * TY;
+    if (t < width) base[(long long)y * g.nzc + t] = buf[t * ny + y];
+  }
+}
+
+// ----------------------------------------------------------------- x pass + Ghat
+template <typename T>
+__device__ __forceinline__ void apply_ghat(cplx<T> X[6], T n0, T n1, T n2) {
+  const T r2 = T(1.41421356237309504880168872420969807857);
+  const T ir2 = T(0.70710678118654752440084436210484903928);
+  cplx<T> a0, a1, a2;
+  a0.x = X[0].x * n0 + ir2 * (X[5].x * n1 + X[4].x * n2);
+  a0.y = X[0].y * n0 + ir2 * (X[5].y * n1 + X[4].y * n2);
+  a1.x = X[1].x * n1 + ir2 * (X[5].x * n0 + X[3].x * n2);
+  a1.y = X[1].y * n1 + ir2 * (X[5].y * n0 + X[3].y * n2);
+  a2.x = X[2].x * n2 + ir2 * (X[4].x * n0 + X[3].x * n1);
+  a2.y = X[2].y * n2 + ir2 * (X[4].y * n0 + X[3].y * n1);
+  const cplx<T> na = {a0.x * n0 + a1.x * n1 + a2.x * n2, a0.y * n0 + a1.y * n1 + a2.y * n2};
+  // S_ij = a_i n_j + n_i a_j - na n_i n_j
+  X[0] = {T(2) * a0.x * n0 - na.x * n0 * n0, T(2) * a0.y * n0 - na.y * n0 * n0};
+  X[1] = {T(2) * a1.x * n1 - na.x * n1 * n1, T(2) * a1.y * n1 - na.y * n1 * n1};
+  X[2] = {T(2) * a2.x * n2 - na.x * n2 * n2, T(2) * a2.y * n2 - na.y * n2 * n2};
+  X[3] = {r2 * (a1.x * n2 + a2.x * n1 - na.x * n1 * n2), r2 * (a1.y * n2 + a2.y * n1 - na.y * n1 * n2)};
+  X[4] = {r2 * (a0.x * n2 + a2.x * n0 - na.x * n0 * n2), r2 * (a0.y * n2 + a2.y * n0 - na.y * n0 * n2)};
+  X[5] = {r2 * (a0.x * n1 + a1.x * n0 - na.x * n0 * n1), r2 * (a0.y * n1 + a1.y * n0 - na.y * n0 * n1)};
+}
+
+template <typename T>
+__device__ __forceinline__ T rnorm3(T a, T b, T c);
+template <>
+__device__ __forceinline__ float rnorm3<float>(float a, float b, float c) {
+  return 1.0f / sqrtf(a * a + b * b + c * c);
+}
+template <>
+__device__ __forceinline__ double rnorm3<double>(double a, double b, double c) {
+  return 1.0 / sqrt(a * a + b * b + c * c);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(kXBT) k_x_ghat(Grid g, AxisPlan px, const cplx<T>* twx, int TX,
+                                                  int ntiles, const T* tanx, const T* tany,
+                                                  const T* tanz, Vec6 mask,
+                                                  const MinresState* st, cplx<T>* spec) {
+  if (st != nullptr && st->done) return;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + g.nx;
+  const int b = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int tile = b % ntiles;
+  const int y = b / ntiles;
+  const int kz0 = tile * TX;
+  const int width = min(TX, g.nzc - kz0);
+  const int nx = g.nx;
+  const long long xstride = (long long)g.ny * g.nzc;
+  cplx<T>* base = spec + (long long)y * g.nzc + kz0;  // spec[c][0][y][kz0]
+  load_twiddles(tw, twx, nx);
+  const int tot = 6 * nx * TX;
+  for (int i = threadIdx.x; i < tot; i += kXBT) {
+    const int t = i % TX;
+    const int cxp = i / TX;  // c*nx + x
+    const int c = cxp / nx, x = cxp - c * nx;
+    buf[(c * TX + t) * nx + x] =
+        t < width ? base[c * g.nspec + x * xstride + t] : cplx<T>{T(0), T(0)};
+  }
+  __syncthreads();
+  block_fft<T, kXBT, kXPTS>(buf, 6 * TX, px, tw, false);
+
+  const int nyqx = (nx % 2 == 0) ? nx / 2 : -1;
+  const int nyqy = (g.ny % 2 == 0) ? g.ny / 2 : -1;
+  const int nyqz = (g.nz % 2 == 0) ? g.nz / 2 : -1;  // never stored, kept for clarity
+  const T ty = tany[y];
+  for (int i = threadIdx.x; i < nx * TX; i += kXBT) {
+    const int t = i % TX, kx = i / TX;
+    const int kz = kz0 + t;
+    if (t >= width) continue;
+    cplx<T> X[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) X[c] = buf[(c * TX + t) * nx + kx];
+    if (kx == nyqx || y == nyqy || kz == nyqz) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) X[c] = {T(0), T(0)};
+    } else if (kx == 0 && y == 0 && kz == 0) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) X[c] = mask.v[c] != 0.0 ? X[c] : cplx<T>{T(0), T(0)};
+    } else {
+      const T tx = tanx[kx], tz = tanz[kz];
+      const T inv = rnorm3<T>(tx, ty, tz);
+      apply_ghat<T>(X, tx * inv, ty * inv, tz * inv);
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) buf[(c * TX + t) * nx + kx] = X[c];
+  }
+  __syncthreads();
+  block_fft<T, kXBT, kXPTS>(buf, 6 * TX, px, tw, true);
+  for (int i = threadIdx.x; i < tot; i += kXBT) {
+    const int t = i % TX;
+    const int cxp = i / TX;
+    const int c = cxp / nx, x = cxp - c * nx;
+    if (t < width) base[c * g.nspec + x * xstride + t] = buf[(c * TX + t) * nx + x];
+  }
+}
+
+// ----------------------------------------------------------------- z inverse
+template <typename T, int MODE, typename TOut>
+__global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, AxisPlan pz, const cplx<T>* twz,
+                                                     int P, const cplx<T>* spec, TOut* out,
+                                                     T* r1, const T* r2,
+                                                     const MinresState* st, double* partials) {
+  if (MODE == kModeMatvec && st->done) return;
+  extern __shared__ unsigned char smem_raw[];
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  cplx<T>* buf = tw + g.nz;
+  __shared__ double red[kZBT / 64];
+  const int nz = g.nz, nzc = g.nzc;
+  const long long nlines = (long long)g.nx * g.ny;
+  const long long line0 = (long long)blockIdx.x * P;
+  load_twiddles(tw, twz, nz);
+  for (int i = threadIdx.x; i < P * 3 * nz; i += kZBT) {
+    const int li = i / (3 * nz);
+    const int rem = i - li * 3 * nz;
+    const int q = rem / nz, k = rem - q * nz;
+    const long long gl = line0 + li;
+    cplx<T> zv = {T(0), T(0)};
+    if (gl < nlines) {
+      const cplx<T>* sa = spec + (2 * q) * g.nspec + gl * nzc;
+      const cplx<T>* sb = spec + (2 * q + 1) * g.nspec + gl * nzc;
+      if (k == 0) {
+        zv = {sa[0].x, sb[0].x};
+      } else if (k < nzc) {
+        const cplx<T> xa = sa[k], xb = sb[k];
+        zv = {xa.x - xb.y, xa.y + xb.x};
+      } else if (!(nz % 2 == 0 && k == nz / 2)) {
+        const cplx<T> xa = cconj(sa[nz - k]), xb = cconj(sb[nz - k]);
+        zv = {xa.x - xb.y, xa.y + xb.x};
+      }
+    }
+    buf[(li * 3 + q) * nz + k] = zv;
+  }
+  __syncthreads();
+  block_fft<T, kZBT, kZPTS>(buf, 3 * P, pz, tw, true);
+
+  const T invn = T(1.0 / double(g.n));
+  double acc = 0.0;
+  T s = T(1), c1 = T(0);
+  if (MODE == kModeMatvec) {
+    s = T(st->s);
+    c1 = T(st->c_r1);
+  }
+  for (int i = threadIdx.x; i < P * nz; i += kZBT) {
+    const int li = i / nz, z = i - li * nz;
+    const long long gl = line0 + li;
+    if (gl >= nlines) continue;
+    const long long p = gl * nz + z;
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const cplx<T> zv = buf[(li * 3 + q) * nz + z];
+      const T va = zv.x * invn, vb = zv.y * invn;
+      if (MODE == kModeProject) {
+        out[(2 * q) * g.n + p] = TOut(va);
+        out[(2 * q + 1) * g.n + p] = TOut(vb);
+      } else {
+        const long long ia = (2 * q) * g.n + p, ib = (2 * q + 1) * g.n + p;
+        const T ua = va - c1 * r1[ia];
+        const T ub = vb - c1 * r1[ib];
+        r1[ia] = ua;
+        r1[ib] = ub;
+        acc += double(s * r2[ia]) * double(ua) + double(s * r2[ib]) * double(ub);
+      }
+    }
+  }
+  if (MODE == kModeMatvec) {
+    const double tot = block_sum<kZBT>(acc, red);
+    if (threadIdx.x == 0) partials[blockIdx.x] = tot;
+  }
+}
+
+template <typename T>
+std::vector<cplx<T>> twiddle_table(int L) {
+  std::vector<cplx<T>> tw(L);
+  for (int m = 0; m < L; ++m) {
+    const double ang = -2.0 * M_PI * double(m) / double(L);
+    tw[m] = {T(std::cos(ang)), T(std::sin(ang))};
+  }
+  return tw;
+}
+
+template <typename T>
+std::vector<T> tan_table(int n, int count) {
+  // t = tan(pi q) with q = fftfreq(n) (small_strain_fft.py:303-313)
+  std::vector<T> t(count);
+  for (int k = 0; k < count; ++k) {
+    const int kk = (k <= (n - 1) / 2) ? k : k - n;
+    t[k] = T(std::tan(M_PI * double(kk) / double(n)));
+  }
+  return t;
+}
+
+template <typename U>
+U* upload(const std::vector<U>& v) {
+  U* d = nullptr;
+  MTL_HIP(hipMalloc(&d, v.size() * sizeof(U)));
+  MTL_HIP(hipMemcpy(d, v.data(), v.size() * sizeof(U), hipMemcpyHostToDevice));
+  return d;
+}
+
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+  MTL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+}
+
+}  // namespace
+
+template <typename T>
+Spectral<T>::Spectral(int nx, int ny, int nz) {
+  if (nx < 1 || ny < 1 || nz < 1) throw Error("grid dimensions must be positive");
+  g_.nx = nx;
+  g_.ny = ny;
+  g_.nz = nz;
+  g_.nzc = (nz + 1) / 2;
+  g_.n = (long long)nx * ny * nz;
+  g_.nspec = (long long)nx * ny * g_.nzc;
+  px_ = make_axis_plan(nx);
+  py_ = make_axis_plan(ny);
+  pz_ = make_axis_plan(nz);
+  const int zcap = kZBT * kZPTS;
+  if (3 * nz > zcap) throw Error("nz too large for the z pass (max 1365)");
+  pz_lines_ = zcap / (3 * nz);
+  const long long nlines = (long long)nx * ny;
+  if (pz_lines_ > nlines) pz_lines_ = int(nlines);
+  if (ny > kYBT * kYPTS) throw Error("ny too large for the y pass");
+  ty_ = std::min({16, (kYBT * kYPTS) / ny, g_.nzc});
+  if (6 * nx > kXBT * kXPTS) throw Error("nx too large for the x pass");
+  tx_ = std::min({8, (kXBT * kXPTS) / (6 * nx), g_.nzc});
+  MTL_HIP(hipMalloc(&spec_, 6 * g_.nspec * sizeof(cplx<T>)));
+  twx_ = upload(twiddle_table<T>(nx));
+  twy_ = upload(twiddle_table<T>(ny));
+  twz_ = upload(twiddle_table<T>(nz));
+  tanx_ = upload(tan_table<T>(nx, nx));
+  tany_ = upload(tan_table<T>(ny, ny));
+  tanz_ = upload(tan_table<T>(nz, g_.nzc));
+}
+
+template <typename T>
+Spectral<T>::~Spectral() {
+  for (void* p : {(void*)spec_, (void*)twx_, (void*)twy_, (void*)twz_, (void*)tanx_,
+                  (void*)tany_, (void*)tanz_})
+    if (p) (void)hipFree(p);
+}
+
+template <typename T>
+int Spectral<T>::zinv_blocks() const {
+  return ceil_div((long long)g_.nx * g_.ny, pz_lines_);
+}
+
+template <typename T>
+template <int MODE, typename TIn, typename TOut>
+void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const Vec6& a, double sgn,
+                      const Vec6& mask, const MinresState* st, double* partials,
+                      hipStream_t s) {
+  const int P = pz_lines_;
+  const int zblocks = zinv_blocks();
+  const size_t zlds = (size_t(g_.nz) + size_t(3 * P) * g_.nz) * sizeof(cplx<T>);
+  auto kzf = k_z_forward<T, MODE, TIn>;
+  allow_lds(kzf, zlds);
+  hipLaunchKernelGGL(kzf, dim3(zblocks), dim3(kZBT), zlds, s, g_, pz_, twz_, P, in, tangent, a,
+                     sgn, st, spec_);
+  MTL_LAUNCH_CHECK();
+
+  const MinresState* stp = MODE == kModeMatvec ? st : nullptr;
+  const int ytiles = ceil_div(g_.nzc, ty_);
+  const size_t ylds = (size_t(g_.ny) + size_t(ty_) * g_.ny) * sizeof(cplx<T>);
+  auto ky = k_y_pass<T>;
+  allow_lds(ky, ylds);
+  hipLaunchKernelGGL(ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, py_, twy_, ty_, ytiles,
+                     false, stp, spec_);
+  MTL_LAUNCH_CHECK();
+
+  const int xtiles = ceil_div(g_.nzc, tx_);
+  const size_t xlds = (size_t(g_.nx) + size_t(6 * tx_) * g_.nx) * sizeof(cplx<T>);
+  auto kx = k_x_ghat<T>;
+  allow_lds(kx, xlds);
+  hipLaunchKernelGGL(kx, dim3(g_.ny * xtiles), dim3(kXBT), xlds, s, g_, px_, twx_, tx_, xtiles,
+                     tanx_, tany_, tanz_, mask, stp, spec_);
+  MTL_LAUNCH_CHECK();
+
+  hipLaunchKernelGGL(ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, py_, twy_, ty_, ytiles,
+                     true, stp, spec_);
+  MTL_LAUNCH_CHECK();
+
+  auto kzi = k_z_inverse<T, MODE, TOut>;
+  allow_lds(kzi, zlds);
+  hipLaunchKernelGGL(kzi, dim3(zblocks), dim3(kZBT), zlds, s, g_, pz_, twz_, P, spec_, out, r1,
+                     reinterpret_cast<const T*>(in), st, partials);
+  MTL_LAUNCH_CHECK();
+}
+
+template <typename T>
+template <typename TIn, typename TOut>
+void Spectral<T>::project(const TIn* in, TOut* out, const Vec6& a, double sgn, const Vec6& mask,
+                          hipStream_t s) {
+  run<kModeProject, TIn, TOut>(in, nullptr, out, nullptr, a, sgn, mask, nullptr, nullptr, s);
+}
+
+template <typename T>
+int Spectral<T>::matvec_minres(const T* r2, const T* tangent, T* r1, const Vec6& mask,
+                               const MinresState* st, double* partials, hipStream_t s) {
+  Vec6 zero{};
+  run<kModeMatvec, T, T>(r2, tangent, static_cast<T*>(nullptr), r1, zero, 1.0, mask, st,
+                         partials, s);
+  return zinv_blocks();
+}
+
+template class Spectral<float>;
+template class Spectral<double>;
+template void Spectral<double>::project<double, double>(const double*, double*, const Vec6&,
+                                                        double, const Vec6&, hipStream_t);
+template void Spectral<float>::project<double, float>(const double*, float*, const Vec6&, double,
+                                                      const Vec6&, hipStream_t);
+template void Spectral<float>::project<double, double>(const double*, double*, const Vec6&,
+                                                       double, const Vec6&, hipStream_t);
+
+}  // namespace mtl

@@ -1,0 +1,51 @@
+"""Elasto-viscoplastic crystal-plasticity model
+(``small_strain_fft/elastic_viscoplastic.py:20-294``).
+
+The per-point implicit update, its masked iteration order and the consistent
+tangent run in the HIP constitutive kernel; this class carries the parameters
+and names the state variables the solver can return.
+"""
+
+import numpy as np
+
+from ...tensor import Order2SymmetricTensor, Scalar, Vector
+from .hardening_laws import KERNEL_LAWS
+
+
+class ElasticViscoplastic:
+    def __init__(self, stiffness, slip_systems, reference_slip_rate, rate_exponent,
+                 slip_resistance, hardening_function, hardening_properties):
+        if hardening_function not in KERNEL_LAWS:
+            raise TypeError(
+                "hardening_function must be one of perfect_plasticity, linear, voce, "
+                "armstrong_frederick (the laws built into the device kernel)")
+        self.stiffness = stiffness
+        self._ref_modulus = float(stiffness.directional_modulus(Vector([1, 0, 0])).components)
+        self.slip_systems = slip_systems
+        self.reference_slip_rate = Scalar(reference_slip_rate)
+        self.rate_exponent = Scalar(rate_exponent)
+        self.initial_slip_resistance = slip_resistance
+        self.hardening_function = hardening_function
+        self.hardening_properties = hardening_properties
+        self.state_variables = dict()
+        self.available_state_variables = [
+            "slip_resistances",
+            "plastic_strains",
+            "slip_system_shear_strains",
+        ]
+        n = len(slip_systems)
+        self._state_variable_info = dict(zip(self.available_state_variables,
+                                             [(Scalar, n), (Order2SymmetricTensor, 0), (Scalar, n)]))
+
+    def kernel_hardening(self):
+        kind, names = KERNEL_LAWS[self.hardening_function]
+        props = self.hardening_properties or {}
+        return kind, [float(props[k]) for k in names]
+
+    def __repr__(self):
+        return (f"ElasticViscoplastic(stiffness={self.stiffness.voigt}, "
+                f"reference_slip_rate={self.reference_slip_rate.components}, "
+                f"rate_exponent={self.rate_exponent.components}, "
+                f"initial_slip_resistance={self.initial_slip_resistance}, "
+                f"hardening_function={self.hardening_function.__name__}, "
+                f"hardening_properties={self.hardening_properties})")
