@@ -1,0 +1,267 @@
+"""Benchmark: SmallStrainFFT on the 128^3 elasto-viscoplastic FCC workload.
+
+Workload (one GPU): 128^3 periodic grid, 1000 periodic-Voronoi grains with
+Haar-random orientations, cubic elasticity C11/C12/C44 = 250/150/120 GPa,
+12 {111}<110> slip systems, power law (reference slip rate 1e-3/s, n = 20,
+g0 = 50 MPa), Voce hardening (tau_sat = 150 MPa, theta_0 = 500 MPa), uniaxial
+x strain rate 1e-3/s, dt = 0.2 s (50 increments to 1 % strain).
+
+A "step" is one load increment (predictor, Newton iterations, MINRES
+corrections, constitutive updates) of that solve.  W untimed increments run
+first, then K increments are timed.  value = voxel-iterations/s =
+N * (Krylov iterations in the timed increments) / wall time; inputs are
+resident in HBM before the timed region.
+
+The roofline entry is the dominant kernel's algorithmic HBM bytes per launch
+divided by its mean launch time (HIP events recorded by the library on the
+launching stream during the timed region), against 8 TB/s.
+
+cpu_baseline: the oracle (NumPy FP64 restatement of the reference path) timed
+on this host for a bounded number of MINRES iterations of the first Newton
+correction of the same 128^3 workload, single-threaded.
+"""
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SEED = 20260003
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=8)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--grid", type=int, default=128)
+    ap.add_argument("--grains", type=int, default=1000)
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--compare-fp64", type=int, default=1,
+                    help="also time the plain FP64 form on the same increments")
+    ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--cpu-iterations", type=int, default=4, help=argparse.SUPPRESS)
+    return ap.parse_args()
+
+
+def workload(n, grains):
+    import numpy as np
+
+    from materialite_amd.microstructure import grain_rotations, periodic_voronoi
+
+    rng = np.random.default_rng(SEED)
+    grain_of_point, _ = periodic_voronoi([n, n, n], grains, rng)
+    rot = grain_rotations(grains, rng)
+    return grain_of_point, rot
+
+
+def build_material(n, grains):
+    import numpy as np
+
+    from materialite_amd import Material, Orientation
+
+    grain_of_point, rot = workload(n, grains)
+    m = Material(dimensions=[n, n, n]).create_fields({"grain": grain_of_point})
+    return m.create_regional_fields(
+        "grain", {"grain": np.arange(grains), "orientation": Orientation(rot, "p")})
+
+
+def evp_model():
+    from materialite_amd import Order4SymmetricTensor, SlipSystem
+    from materialite_amd.models.small_strain_fft import ElasticViscoplastic, voce
+
+    c = Order4SymmetricTensor.from_cubic_constants(250e3, 150e3, 120e3)  # MPa
+    return ElasticViscoplastic(c, SlipSystem.octahedral(), reference_slip_rate=1e-3,
+                               rate_exponent=20.0, slip_resistance=50.0, hardening_function=voce,
+                               hardening_properties={"tau_1": 100.0, "theta_0": 500.0,
+                                                     "theta_1": 0.0})
+
+
+def time_increments(material, precision, warmup, steps, profile):
+    from materialite_amd import _native
+    from materialite_amd.models.small_strain_fft import LoadSchedule, SmallStrainFFT
+
+    ls = LoadSchedule.from_constant_uniaxial_strain_rate(1e-3, "x")
+    model = SmallStrainFFT(ls, end_time=10.0, initial_time_increment=0.2,
+                           constitutive_model=evp_model(), precision=precision)
+    rec = {"krylov": 0, "newton": 0, "t0": None, "t1": None, "steps": 0}
+
+    def cb(info):
+        # every increment ends with a host synchronisation inside the library,
+        # so the device is idle whenever this runs
+        print(f"[bench] {precision} increment {info['step']}: {info['newton_iterations']} Newton, "
+              f"{info['krylov_iterations']} Krylov, {info['constitutive_iterations']} constitutive "
+              f"iterations", file=sys.stderr, flush=True)
+        if info["step"] == warmup:
+            if profile:
+                _native.profile_reset()
+                _native.profile_enable(True)
+            rec["t0"] = time.perf_counter()
+        elif info["step"] > warmup:
+            rec["krylov"] += info["krylov_iterations"]
+            rec["newton"] += info["newton_iterations"]
+            rec["steps"] += 1
+        if info["step"] == warmup + steps:
+            rec["t1"] = time.perf_counter()
+            return True
+        return False
+
+    if warmup == 0:
+        raise SystemExit("--warmup must be >= 1 (the first increment builds the solver state)")
+    model.run(material, increment_callback=cb)
+    if profile:
+        rec["profile"] = _native.profile_read()
+        _native.profile_enable(False)
+    rec["seconds"] = rec["t1"] - rec["t0"]
+    return rec
+
+
+def kernel_bytes(n, nzc_frac, precision):
+    """Algorithmic HBM bytes per launch of each pass kernel (FFT/Krylov type)."""
+    esz = 4 if precision == "mixed" else 8
+    spec = 6 * n * nzc_frac * 2 * esz  # one read or write of the 6-component half spectrum
+    return {
+        "z_forward": n * (6 + 21) * esz + spec,  # r2 (6) and tangent (21) read, spectrum written
+        "y_pass": 2 * spec,
+        "x_ghat": 2 * spec,
+        "z_inverse": spec + n * 3 * 6 * esz,  # spectrum, r1 and r2 read, u written
+    }
+
+
+def cpu_baseline_child(n, grains, iters):
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    import numpy as np
+
+    from oracle import tensors as T
+    from oracle.minres import minres
+    from oracle.projection import apply_projection
+
+    grain_of_point, rot = workload(n, grains)
+    npts = n ** 3
+    c = T.cubic_stiffness(250e3, 150e3, 120e3)
+    rm = T.rotation_matrix_mandel(rot)
+    tangent_g = T.order4_to_specimen(np.broadcast_to(c, rm.shape), rm)
+    tangent = tangent_g[grain_of_point]
+    mask = np.array([0, 1, 1, 1, 1, 1.0])
+    d_strain = np.array([2e-4, 0, 0, 0, 0, 0])
+    sigma = np.einsum("pmn,n->pm", tangent, d_strain)
+    b = apply_projection(-sigma, (n, n, n), stress_mask=mask).ravel()
+
+    def lhs(x):
+        return apply_projection(np.einsum("pmn,pn->pm", tangent, x.reshape(npts, 6)), (n, n, n),
+                                stress_mask=mask).ravel()
+
+    lhs(b)  # warm caches / page in
+    t0 = time.perf_counter()
+    _, itn, _ = minres(lhs, b, rtol=1e-30, maxiter=iters)
+    dt = time.perf_counter() - t0
+    print(json.dumps({"vi_per_s": npts * itn / dt, "iterations": itn, "seconds": dt}))
+
+
+def cpu_baseline(n, grains, iters):
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        env[k] = "1"
+    cmd = [sys.executable, os.path.abspath(__file__), "--cpu-baseline-child", "--grid", str(n),
+           "--grains", str(grains), "--cpu-iterations", str(iters)]
+    r = subprocess.run(cmd, capture_output=True, text=True, env=env, timeout=900)
+    if r.returncode != 0:
+        raise RuntimeError(r.stderr[-2000:])
+    res = json.loads(r.stdout.strip().splitlines()[-1])
+    return {
+        "value": res["vi_per_s"],
+        "unit": "voxel-iters/s",
+        "cores": 1,
+        "kind": "port",
+        "sample": (f"oracle FP64 NumPy MINRES, {res['iterations']} Krylov iterations of the first "
+                   f"Newton correction of the {n}^3 EVP workload ({res['seconds']:.1f} s, 1 thread)"),
+    }
+
+
+def main():
+    args = parse()
+    if args.cpu_baseline_child:
+        cpu_baseline_child(args.grid, args.grains, args.cpu_iterations)
+        return
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build has no "
+              "multi-GPU path", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    n = args.grid
+    npts = n ** 3
+    material = build_material(n, args.grains)
+    main_rec = time_increments(material, args.precision, args.warmup, args.steps, profile=True)
+    vi = npts * main_rec["krylov"] / main_rec["seconds"]
+    prof = main_rec["profile"]
+    nzc_frac = ((n + 1) // 2) / n
+    kb = kernel_bytes(npts, nzc_frac, args.precision)
+    best = max(kb, key=lambda k: prof[k][0])
+    ms, launches = prof[best]
+    # launches that did work: one per Krylov iteration (two for the y pass, run
+    # forward and inverse); launches enqueued past MINRES convergence return at
+    # once and only add their few microseconds to the total (conservative)
+    real = main_rec["krylov"] * (2 if best == "y_pass" else 1)
+    mean_s = ms / 1e3 / max(real, 1)
+    achieved = kb[best] / mean_s / 1e9
+    fp64 = None
+    if args.compare_fp64 and args.precision != "fp64":
+        rec = time_increments(material, "fp64", args.warmup, args.steps, profile=False)
+        fp64 = {
+            "value": npts * rec["krylov"] / rec["seconds"],
+            "krylov_iterations": rec["krylov"],
+            "seconds": rec["seconds"],
+            "ms_per_step": 1e3 * rec["seconds"] / max(rec["steps"], 1),
+        }
+    cpu = cpu_baseline(n, args.grains, args.cpu_iterations) if args.cpu_baseline else None
+    line = {
+        "metric": "voxel-iters/s at 128³ EVP, FP32-refined vs FP64; p50/p99 solve latency mixed",
+        "value": vi,
+        "unit": "voxel-iters/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * main_rec["seconds"] / max(main_rec["steps"], 1),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32" if args.precision == "mixed" else "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{n}^3 EVP FCC, {args.grains} periodic Voronoi grains, Voce, uniaxial x "
+                        f"1e-3/s, dt 0.2 s, increments {args.warmup + 1}-{args.warmup + args.steps}",
+            "precision": args.precision,
+            "krylov_iterations": main_rec["krylov"],
+            "newton_iterations": main_rec["newton"],
+            "seed": SEED,
+        },
+        "roofline": {
+            "bound": "hbm",
+            "kernel": best,
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "mean_launch_us": mean_s * 1e6,
+        },
+        "kernels_ms": {k: round(v[0], 3) for k, v in prof.items()},
+        "fp64_form": fp64,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -131,6 +131,18 @@ int mtl_solver_minres(mtl_solver* h, const double* b_dev, double* x_dev,
 int mtl_solver_constitutive(mtl_solver* h, const double* strain_dev, const double* guess_dev,
                             double time_increment, double* stress_dev, int* iterations);
 
+/* ---- kernel timing (no counterpart in the reference, which has no profiler
+ * hooks; used by bench.py for the roofline figures) ----
+ * HIP events are recorded around each library kernel on its launch stream
+ * while enabled.  kernel ids: 0 z-forward FFT pass, 1 y FFT pass, 2 x FFT pass
+ * with Ghat, 3 z-inverse FFT pass, 4 MINRES vector updates, 5 MINRES scalar
+ * steps, 6 EVP iteration, 7 EVP outputs, 8 elastic update, 9 Newton
+ * reductions, 10-13 the four FFT passes (ids 0-3) when they project a
+ * residual instead of applying the Krylov matvec. */
+int mtl_profile_enable(int enable);
+int mtl_profile_reset(void);
+int mtl_profile_read(int kernel, double* total_ms, long long* launches);
+
 #ifdef __cplusplus
 }
 #endif

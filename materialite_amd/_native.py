@@ -94,7 +94,35 @@ EXPORTS = {
     "mtl_solver_constitutive": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_double, ctypes.c_void_p,
                                                ctypes.POINTER(ctypes.c_int)]),
+    "mtl_profile_enable": (ctypes.c_int, [ctypes.c_int]),
+    "mtl_profile_reset": (ctypes.c_int, []),
+    "mtl_profile_read": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_longlong)]),
 }
+
+KERNELS = ["z_forward", "y_pass", "x_ghat", "z_inverse", "minres_vector", "minres_scalar",
+           "evp_iteration", "evp_outputs", "elastic", "newton", "residual_z_forward",
+           "residual_y_pass", "residual_x_ghat", "residual_z_inverse"]
+
+
+def profile_enable(on=True):
+    check(load_library().mtl_profile_enable(1 if on else 0))
+
+
+def profile_reset():
+    check(load_library().mtl_profile_reset())
+
+
+def profile_read():
+    """{kernel name: (total ms, launches)} since the last reset."""
+    lib = load_library()
+    out = {}
+    for i, name in enumerate(KERNELS):
+        ms = ctypes.c_double()
+        n = ctypes.c_longlong()
+        check(lib.mtl_profile_read(i, ctypes.byref(ms), ctypes.byref(n)))
+        out[name] = (ms.value, n.value)
+    return out
 
 _LIB = None
 

@@ -20,9 +20,12 @@
 // masked phase and the outputs run in one kernel with the state in registers.
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 #include "constitutive.hpp"
+#include "profiler.hpp"
 
 namespace mtl {
 
@@ -163,11 +166,14 @@ struct EvpParams {
 };
 
 struct EvpControl {
-  int count[kEvpMaxIterations + 4];
-  int stopped;
-  int kstar;
+  int count[kEvpMaxIterations + 4];  // points converging in iteration it
+  long long total;                   // points converged after the last iteration
+  int masked;                        // masked phase active
+  int kstar;                         // iteration that started it
+  int done;
   int fail;
-  int extra;
+  int iterations;
+  int pad;
 };
 
 __device__ __forceinline__ double powi_abs(double x, int e, double ef) {
@@ -320,6 +326,7 @@ struct EvpBuffers {
   double* sig[2];    // [6][n] ping-pong
   double* g[2];      // [NS][n] ping-pong
   unsigned char* conv;
+  int* last;         // [n] last iteration each point ran
   double* ep_old;    // [6][n]
   double* g_old;     // [NS][n]
   double* gam_old;   // [NS][n]
@@ -383,17 +390,21 @@ __device__ __forceinline__ void load_point(const EvpBuffers& B, int n, int i, in
   acc_old = B.acc_old[i];
 }
 
-// unmasked iteration `it` (1-based): reads buffer (it-1)&1, writes it&1
+// Iteration `it` (1-based) of every point that still iterates: all points
+// before the masked phase, only unconverged ones after it
+// (elastic_viscoplastic.py:116-158).  A point's state after iteration j lives
+// in ping-pong buffer j&1; last[i] records the last iteration it ran.
 template <int NS>
-__global__ void __launch_bounds__(kCBT) k_evp_iter_all(int it, int n, EvpParams<NS> q,
-                                                        EvpBuffers B, EvpControl* ctrl) {
-  if (ctrl->stopped) return;
+__global__ void __launch_bounds__(kCBT) k_evp_iter(int it, int n, EvpParams<NS> q, EvpBuffers B,
+                                                    EvpControl* ctrl) {
+  if (ctrl->done) return;
+  const int masked = ctrl->masked;
   __shared__ int cnt;
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   const int i = blockIdx.x * kCBT + threadIdx.x;
-  const int src = (it - 1) & 1, dst = it & 1;
-  if (i < n) {
+  if (i < n && !(masked && B.conv[i])) {
+    const int src = (it - 1) & 1, dst = it & 1;
     double sig[6], g[NS], eps[6], ep_old[6], g_old[NS], acc_old;
     load_point<NS>(B, n, i, src, sig, g, eps, ep_old, g_old, acc_old);
     double sig_new[6], g_new[NS], ep[6], dgam[NS], Sinv[21], diff;
@@ -405,61 +416,49 @@ __global__ void __launch_bounds__(kCBT) k_evp_iter_all(int it, int n, EvpParams<
     for (int s = 0; s < NS; ++s) B.g[dst][s * ln + i] = g_new[s];
     const bool cv = evp_converged<NS>(q, diff, B.gnorm[i]);
     B.conv[i] = cv ? 1 : 0;
+    B.last[i] = it;
     if (cv) atomicAdd(&cnt, 1);
   }
   __syncthreads();
   if (threadIdx.x == 0 && cnt) atomicAdd(&ctrl->count[it], cnt);
 }
 
+// Convergence bookkeeping after iteration `it` (elastic_viscoplastic.py:160-170).
 __global__ void k_evp_control(int it, int n, EvpControl* ctrl) {
-  if (threadIdx.x != 0 || ctrl->stopped) return;
-  const long long c = ctrl->count[it];
-  if (2 * c >= n) {  // converged fraction >= 0.5 (elastic_viscoplastic.py:116)
-    ctrl->kstar = it;
-    ctrl->stopped = 1;
-  } else if (it >= kEvpMaxIterations) {
+  if (threadIdx.x != 0 || ctrl->done) return;
+  // unmasked: every point was tested; masked: converged points kept their flag
+  const long long total = ctrl->masked ? ctrl->total + ctrl->count[it] : ctrl->count[it];
+  ctrl->total = total;
+  if (total == n) {
+    ctrl->done = 1;
+    ctrl->iterations = it;
+    return;
+  }
+  if (it >= kEvpMaxIterations) {
     ctrl->fail = 1;
-    ctrl->stopped = 1;
+    ctrl->done = 1;
+    return;
+  }
+  if (!ctrl->masked && 2 * total >= n) {  // converged fraction >= 0.5
+    ctrl->masked = 1;
+    ctrl->kstar = it;
   }
 }
 
+// Outputs of each point's last iteration, recomputed from the state it started
+// from (bitwise the same stress and slip resistances as that iteration wrote).
 template <int NS, typename K>
-__global__ void __launch_bounds__(kCBT) k_evp_finish(int n, const int* idx, long long npts,
-                                                      const double* rot9, EvpParams<NS> q,
-                                                      EvpBuffers B, EvpControl* ctrl, double* sig_out,
-                                                      K* tangent) {
+__global__ void __launch_bounds__(kCBT) k_evp_outputs(int n, const int* idx, long long npts,
+                                                       const double* rot9, EvpParams<NS> q,
+                                                       EvpBuffers B, const EvpControl* ctrl,
+                                                       double* sig_out, K* tangent) {
   const int i = blockIdx.x * kCBT + threadIdx.x;
-  if (i >= n || ctrl->fail) return;
-  const int kstar = ctrl->kstar;
-  const int after = kstar & 1, before = (kstar - 1) & 1;
+  if (i >= n || ctrl->fail || !ctrl->done) return;
+  const int src = (B.last[i] - 1) & 1;
   double sig[6], g[NS], eps[6], ep_old[6], g_old[NS], acc_old;
   double sig_new[6], g_new[NS], ep[6], dgam[NS], Sinv[21], diff;
-  int extra = 0;
-  if (B.conv[i]) {
-    // converged at K*: recompute iteration K* from its starting state to get
-    // its plastic strain, slip increments and tangent (bitwise the same sig/g)
-    load_point<NS>(B, n, i, before, sig, g, eps, ep_old, g_old, acc_old);
-    evp_iterate<NS>(q, sig, g, eps, ep_old, g_old, acc_old, sig_new, g_new, ep, dgam, Sinv, diff);
-  } else {
-    load_point<NS>(B, n, i, after, sig, g, eps, ep_old, g_old, acc_old);
-    const double gn = B.gnorm[i];
-    bool cv = false;
-    for (int it = kstar + 1; it <= kEvpMaxIterations; ++it) {
-      evp_iterate<NS>(q, sig, g, eps, ep_old, g_old, acc_old, sig_new, g_new, ep, dgam, Sinv, diff);
-      ++extra;
-      cv = evp_converged<NS>(q, diff, gn);
-      if (cv) break;
-#pragma unroll
-      for (int c = 0; c < 6; ++c) sig[c] = sig_new[c];
-#pragma unroll
-      for (int s = 0; s < NS; ++s) g[s] = g_new[s];
-    }
-    if (!cv) {
-      atomicOr(&ctrl->fail, 1);
-      return;
-    }
-    atomicMax(&ctrl->extra, extra);
-  }
+  load_point<NS>(B, n, i, src, sig, g, eps, ep_old, g_old, acc_old);
+  evp_iterate<NS>(q, sig, g, eps, ep_old, g_old, acc_old, sig_new, g_new, ep, dgam, Sinv, diff);
   const long long p = idx[i];
   const long long ln = n;
   double Rm[36], vs[6], Ds[21];
@@ -480,6 +479,7 @@ __global__ void __launch_bounds__(kCBT) k_evp_finish(int n, const int* idx, long
   }
   B.acc_pend[i] = acc_old + asum;
 }
+
 
 // crystal plastic strain -> specimen frame, AoS output for the host
 __global__ void __launch_bounds__(kCBT) k_ep_specimen(int n, const int* idx, long long npts,
@@ -575,9 +575,11 @@ class ElasticModel final : public Model<K> {
   void launch(const double* rot9, const double* eps, double* sig, K* tangent, int ws,
               hipStream_t s) {
     if (this->n_ == 0) return;
+    profiler().begin(s);
     hipLaunchKernelGGL(k_elastic<K>, dim3(ceil_div(this->n_, kCBT)), dim3(kCBT), 0, s, this->n_,
                        this->idx_, this->npts_, rot9, C_, eps, sig, tangent, ws);
     MTL_LAUNCH_CHECK();
+    profiler().end(kKElastic, s);
   }
   Stiff21 C_;
 };
@@ -615,6 +617,7 @@ class EvpModel final : public Model<K> {
       alloc(B_.g[k], NS * n);
     }
     MTL_HIP(hipMalloc(&B_.conv, n));
+    MTL_HIP(hipMalloc(&B_.last, n * sizeof(int)));
     alloc(B_.ep_old, 6 * n);
     alloc(B_.g_old, NS * n);
     alloc(B_.gam_old, NS * n);
@@ -636,6 +639,7 @@ class EvpModel final : public Model<K> {
                       B_.acc_pend, ep_tr_, g_tr_, gam_tr_, acc_tr_})
       if (p) (void)hipFree(p);
     if (B_.conv) (void)hipFree(B_.conv);
+    if (B_.last) (void)hipFree(B_.last);
     if (ctrl_) (void)hipFree(ctrl_);
     if (hctrl_) (void)hipHostFree(hctrl_);
     (void)hipFree(this->idx_);
@@ -669,34 +673,44 @@ class EvpModel final : public Model<K> {
     MTL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_evp_ctrl_reset, dim3(1), dim3(64), 0, s, ctrl_);
     MTL_LAUNCH_CHECK();
+    // iterations are enqueued in chunks; the kernels return at once after
+    // convergence, and the host reads the control block once per chunk
     int it = 0;
-    const int chunk = 4;
+    int chunk = 6;
     while (true) {
       for (int k = 0; k < chunk && it < kEvpMaxIterations; ++k) {
         ++it;
-        hipLaunchKernelGGL(k_evp_iter_all<NS>, dim3(blocks), dim3(kCBT), 0, s, it, n, q_, B_,
-                           ctrl_);
+        profiler().begin(s);
+        hipLaunchKernelGGL(k_evp_iter<NS>, dim3(blocks), dim3(kCBT), 0, s, it, n, q_, B_, ctrl_);
         MTL_LAUNCH_CHECK();
+        profiler().end(kKEvpIter, s);
         hipLaunchKernelGGL(k_evp_control, dim3(1), dim3(64), 0, s, it, n, ctrl_);
         MTL_LAUNCH_CHECK();
       }
       MTL_HIP(hipMemcpyAsync(hctrl_, ctrl_, sizeof(EvpControl), hipMemcpyDeviceToHost, s));
       MTL_HIP(hipStreamSynchronize(s));
-      if (hctrl_->stopped || it >= kEvpMaxIterations) break;
+      if (hctrl_->done || it >= kEvpMaxIterations) break;
+      chunk = 4;
     }
-    if (hctrl_->fail || !hctrl_->stopped) return 0;
-    hipLaunchKernelGGL((k_evp_finish<NS, K>), dim3(blocks), dim3(kCBT), 0, s, n, this->idx_,
+    if (std::getenv("MTL_DEBUG") != nullptr) {
+      std::fprintf(stderr, "[mtl] evp n=%d converging per iteration:", n);
+      for (int k = 1; k <= std::min(it, kEvpMaxIterations); ++k)
+        std::fprintf(stderr, " %d", hctrl_->count[k]);
+      std::fprintf(stderr, " | K*=%d iterations=%d fail=%d\n", hctrl_->kstar, hctrl_->iterations,
+                   hctrl_->fail);
+    }
+    if (hctrl_->fail || !hctrl_->done) return 0;
+    profiler().begin(s);
+    hipLaunchKernelGGL((k_evp_outputs<NS, K>), dim3(blocks), dim3(kCBT), 0, s, n, this->idx_,
                        this->npts_, rot9, q_, B_, ctrl_, sig, tangent);
     MTL_LAUNCH_CHECK();
-    MTL_HIP(hipMemcpyAsync(hctrl_, ctrl_, sizeof(EvpControl), hipMemcpyDeviceToHost, s));
-    MTL_HIP(hipStreamSynchronize(s));
-    if (hctrl_->fail) return 0;
+    profiler().end(kKEvpFinish, s);
     // adopt the call's outputs as the trial state (elastic_viscoplastic.py:175-182)
     std::swap(B_.ep_pend, ep_tr_);
     std::swap(B_.g_pend, g_tr_);
     std::swap(B_.gam_pend, gam_tr_);
     std::swap(B_.acc_pend, acc_tr_);
-    return hctrl_->kstar + hctrl_->extra;
+    return hctrl_->iterations;
   }
 
   void commit(const double*, hipStream_t s) override {

@@ -11,8 +11,11 @@
 // and the host polls `done` once per chunk of iterations.
 #include <cfloat>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 
 #include "minres.hpp"
+#include "profiler.hpp"
 
 namespace mtl {
 
@@ -28,14 +31,17 @@ __device__ double sum_partials(const double* partials, int count, double* red) {
   return block_sum<kVBT>(acc, red);
 }
 
+// r1 is zeroed although iteration 1 multiplies it by 0 (c_r1 = 0): stale
+// memory may hold inf/NaN bit patterns and 0 * inf would poison u.
 template <typename T>
-__global__ void __launch_bounds__(kVBT) k_init(long long n6, const double* b, T* r2, T* x, T* w,
-                                                T* w2, double* partials) {
+__global__ void __launch_bounds__(kVBT) k_init(long long n6, const double* b, T* r1, T* r2, T* x,
+                                                T* w, T* w2, double* partials) {
   __shared__ double red[kVBT / 64];
   double acc = 0.0;
   for (long long i = blockIdx.x * (long long)kVBT + threadIdx.x; i < n6;
        i += (long long)gridDim.x * kVBT) {
     const T v = T(b[i]);
+    r1[i] = T(0);
     r2[i] = v;
     x[i] = T(0);
     w[i] = T(0);
@@ -221,7 +227,7 @@ int Minres<T>::solve(const double* b, const T* tangent, const Vec6& mask, double
   T* w2 = vec_[3];
   T* wf = vec_[4];
   const int vb = vec_blocks(n6_);
-  hipLaunchKernelGGL(k_init<T>, dim3(vb), dim3(kVBT), 0, s, n6_, b, r2, x_, w2, wf, partials_);
+  hipLaunchKernelGGL(k_init<T>, dim3(vb), dim3(kVBT), 0, s, n6_, b, r1, r2, x_, w2, wf, partials_);
   MTL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_start, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb, rtol, maxiter);
   MTL_LAUNCH_CHECK();
@@ -229,24 +235,36 @@ int Minres<T>::solve(const double* b, const T* tangent, const Vec6& mask, double
   std::swap(w1, wf);
 
   int chunk = 8;
+  int prev_itn = 0;
+  double prev_test1 = 0.0;
   long long launched = 0;
   while (true) {
     for (int it = 0; it < chunk; ++it) {
       const int np = sp_.matvec_minres(r2, tangent, r1, mask, state_, partials_, s);
+      profiler().begin(s);
       hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kVBT), 0, s, state_, partials_, np);
       MTL_LAUNCH_CHECK();
+      profiler().end(kKMinresScalar, s);
+      profiler().begin(s);
       hipLaunchKernelGGL(k_lanczos<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, r2, partials_);
       MTL_LAUNCH_CHECK();
+      profiler().end(kKMinresVector, s);
+      profiler().begin(s);
       hipLaunchKernelGGL(k_givens, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb);
       MTL_LAUNCH_CHECK();
+      profiler().end(kKMinresScalar, s);
       // Lanczos roles rotate: r1 <- r2 (old), r2 <- y (stored in old r1)
       std::swap(r1, r2);
       // v_k = r2_old / beta_k now lives in r1
+      profiler().begin(s);
       hipLaunchKernelGGL(k_update<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, w1, w2, wf, x_,
                          partials_);
       MTL_LAUNCH_CHECK();
+      profiler().end(kKMinresVector, s);
+      profiler().begin(s);
       hipLaunchKernelGGL(k_tests, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb);
       MTL_LAUNCH_CHECK();
+      profiler().end(kKMinresScalar, s);
       // w roles rotate: w1 <- w2, w2 <- new w, free slot <- old w1
       T* old_w1 = w1;
       w1 = w2;
@@ -256,8 +274,23 @@ int Minres<T>::solve(const double* b, const T* tangent, const Vec6& mask, double
     }
     MTL_HIP(hipMemcpyAsync(host_state_, state_, sizeof(MinresState), hipMemcpyDeviceToHost, s));
     MTL_HIP(hipStreamSynchronize(s));
+    if (std::getenv("MTL_DEBUG") != nullptr) {
+      const MinresState& z = *host_state_;
+      std::fprintf(stderr, "[mtl]     minres itn %d beta %.4e alfa %.4e phibar %.4e test1 %.3e test2 %.3e\n",
+                   z.itn, z.beta, z.alfa, z.phibar, z.test1, z.test2);
+    }
     if (host_state_->done || launched >= maxiter) break;
-    if (chunk < 64) chunk *= 2;
+    // size the next chunk from the observed geometric decay of SciPy's test1
+    // so that few iterations are enqueued past convergence
+    const MinresState& z = *host_state_;
+    chunk = 8;
+    if (prev_itn > 0 && z.itn > prev_itn && z.test1 > 0.0 && prev_test1 > z.test1) {
+      const double rate = std::pow(z.test1 / prev_test1, 1.0 / double(z.itn - prev_itn));
+      const double need = std::log(z.rtol / z.test1) / std::log(rate);
+      if (std::isfinite(need)) chunk = int(std::min(64.0, std::max(2.0, std::ceil(need) + 1.0)));
+    }
+    prev_itn = z.itn;
+    prev_test1 = z.test1;
   }
   if (istop_out) *istop_out = host_state_->istop;
   return host_state_->itn;

@@ -7,6 +7,8 @@
 // (strain, stress, residual) or [21][N] K (tangent, K = float or double).
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <memory>
 #include <string>
@@ -15,6 +17,7 @@
 #include "../../include/materialite_amd.h"
 #include "constitutive.hpp"
 #include "minres.hpp"
+#include "profiler.hpp"
 
 namespace mtl {
 
@@ -152,6 +155,16 @@ __global__ void k_convert(long long count, const From* a, To* b) {
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
        i += (long long)gridDim.x * blockDim.x)
     b[i] = To(a[i]);
+}
+
+constexpr double kMixedKrylovFloor = 1.0e-5;
+
+bool debug_enabled() {
+  static const bool on = [] {
+    const char* e = std::getenv("MTL_DEBUG");
+    return e != nullptr && e[0] != '\0' && e[0] != '0';
+  }();
+  return on;
 }
 
 int grid_for(long long count) {
@@ -334,6 +347,14 @@ class Solver final : public SolverBase {
 
     const int kMaxNewton = 100;  // small_strain_fft.py:125
     const long long maxiter = 5LL * 6LL * n_;
+    // Mixed form: FP32 Krylov solves cannot honour relative tolerances much
+    // below 1e-5, so the inner tolerance is floored there.  The Newton
+    // residual, the equilibrium test and the strain/stress fields stay FP64,
+    // so each Newton iteration is an FP64-residual refinement step and the
+    // converged state is the FP64 one up to the requested global tolerance.
+    constexpr bool kMixed = !std::is_same<K, double>::value;
+    const double lin_tol =
+        kMixed ? std::max(P.linear_solver_tolerance, kMixedKrylovFloor) : P.linear_solver_tolerance;
     bool converged = false;
     int iteration = 0;
     int max_constit = 0;
@@ -356,8 +377,11 @@ class Solver final : public SolverBase {
       }
       old_eq = std::max(eq_err, P.global_tolerance);
       int istop = 0;
-      const int itn = minres_->solve(b_, D_, mask, P.linear_solver_tolerance, maxiter, s_, &istop);
+      const int itn = minres_->solve(b_, D_, mask, lin_tol, maxiter, s_, &istop);
       R.krylov_iterations += itn;
+      if (debug_enabled())
+        std::fprintf(stderr, "[mtl] step %d newton %d: minres %d iterations (istop %d)\n",
+                     P.time_step_id, iteration, itn, istop);
       hipLaunchKernelGGL(k_add_correction<K>, dim3(nb), dim3(kNBT), 0, s_, 6 * n_, minres_->x(), eps_,
                          fluct_);
       MTL_LAUNCH_CHECK();
@@ -382,11 +406,15 @@ class Solver final : public SolverBase {
           (strain_err < P.strain_correction_tolerance ||
            projected < P.strain_correction_tolerance || P.time_step_id == 0 || eq_err < 1e-8))
         converged = true;
+      if (debug_enabled())
+        std::fprintf(stderr, "[mtl]   eq_err %.3e strain_err %.3e projected %.3e -> %s\n", eq_err,
+                     strain_err, projected, converged ? "converged" : "continue");
       copy6(guess_, sig_);
       R.scaled_strain_error = strain_err;
       R.projected_scaled_strain_error = projected;
       if (iteration == kMaxNewton && !converged) break;
     }
+    profiler().flush();
     R.converged = converged ? 1 : 0;
     R.newton_iterations = iteration;
     R.max_constitutive_iterations = max_constit;
@@ -504,6 +532,11 @@ class Solver final : public SolverBase {
   std::vector<ModelDesc> descs_;
   std::vector<std::unique_ptr<Model<K>>> models_;
 };
+
+Profiler& profiler() {
+  static Profiler p;
+  return p;
+}
 
 bool inverse6_host(const double* A, double* out) {
   double m[6][12];
@@ -650,6 +683,27 @@ int mtl_solver_minres(mtl_solver* h, const double* b_dev, double* x_dev,
                       const double stress_mask[6], double rtol, int* iterations) {
   if (!h || !b_dev || !x_dev || !stress_mask || !iterations) return bad("null argument");
   return guarded([&] { *iterations = h->impl->minres(b_dev, x_dev, stress_mask, rtol); });
+}
+
+int mtl_profile_enable(int enable) {
+  return guarded([&] {
+    mtl::profiler().flush();
+    mtl::profiler().enabled = enable != 0;
+  });
+}
+
+int mtl_profile_reset(void) {
+  return guarded([&] { mtl::profiler().reset(); });
+}
+
+int mtl_profile_read(int kernel, double* total_ms, long long* launches) {
+  if (!total_ms || !launches) return bad("null argument");
+  if (kernel < 0 || kernel >= mtl::kNumKernelIds) return bad("unknown kernel id");
+  return guarded([&] {
+    mtl::profiler().flush();
+    *total_ms = mtl::profiler().total_ms[kernel];
+    *launches = mtl::profiler().launches[kernel];
+  });
 }
 
 int mtl_solver_constitutive(mtl_solver* h, const double* strain_dev, const double* guess_dev,

@@ -12,8 +12,11 @@
 // small_strain_fft.py:301-328); zero on even-N Nyquist planes (:354-359) and
 // diag(mask) at k = 0 (:363-367).
 #include <cmath>
+#include <mutex>
+#include <unordered_map>
 #include <vector>
 
+#include "profiler.hpp"
 #include "spectral.hpp"
 
 namespace mtl {
@@ -322,10 +325,17 @@ U* upload(const std::vector<U>& v) {
   return d;
 }
 
+// Raise a kernel's dynamic-LDS limit once to the largest size it is launched with.
 template <typename K>
 void allow_lds(K kernel, size_t bytes) {
-  MTL_HIP(hipFuncSetAttribute(reinterpret_cast<const void*>(kernel),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+  static std::mutex mu;
+  static std::unordered_map<const void*, size_t> granted;
+  const void* key = reinterpret_cast<const void*>(kernel);
+  std::lock_guard<std::mutex> lock(mu);
+  auto it = granted.find(key);
+  if (it != granted.end() && bytes <= it->second) return;
+  MTL_HIP(hipFuncSetAttribute(key, hipFuncAttributeMaxDynamicSharedMemorySize, int(bytes)));
+  granted[key] = bytes;
 }
 
 }  // namespace
@@ -382,36 +392,46 @@ void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const V
   const size_t zlds = (size_t(g_.nz) + size_t(3 * P) * g_.nz) * sizeof(cplx<T>);
   auto kzf = k_z_forward<T, MODE, TIn>;
   allow_lds(kzf, zlds);
+  profiler().begin(s);
   hipLaunchKernelGGL(kzf, dim3(zblocks), dim3(kZBT), zlds, s, g_, pz_, twz_, P, in, tangent, a,
                      sgn, st, spec_);
   MTL_LAUNCH_CHECK();
+  profiler().end(MODE == kModeMatvec ? kKZForward : residual_id(kKZForward), s);
 
   const MinresState* stp = MODE == kModeMatvec ? st : nullptr;
   const int ytiles = ceil_div(g_.nzc, ty_);
   const size_t ylds = (size_t(g_.ny) + size_t(ty_) * g_.ny) * sizeof(cplx<T>);
   auto ky = k_y_pass<T>;
   allow_lds(ky, ylds);
+  profiler().begin(s);
   hipLaunchKernelGGL(ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, py_, twy_, ty_, ytiles,
                      false, stp, spec_);
   MTL_LAUNCH_CHECK();
+  profiler().end(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), s);
 
   const int xtiles = ceil_div(g_.nzc, tx_);
   const size_t xlds = (size_t(g_.nx) + size_t(6 * tx_) * g_.nx) * sizeof(cplx<T>);
   auto kx = k_x_ghat<T>;
   allow_lds(kx, xlds);
+  profiler().begin(s);
   hipLaunchKernelGGL(kx, dim3(g_.ny * xtiles), dim3(kXBT), xlds, s, g_, px_, twx_, tx_, xtiles,
                      tanx_, tany_, tanz_, mask, stp, spec_);
   MTL_LAUNCH_CHECK();
+  profiler().end(MODE == kModeMatvec ? kKXGhat : residual_id(kKXGhat), s);
 
+  profiler().begin(s);
   hipLaunchKernelGGL(ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, py_, twy_, ty_, ytiles,
                      true, stp, spec_);
   MTL_LAUNCH_CHECK();
+  profiler().end(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), s);
 
   auto kzi = k_z_inverse<T, MODE, TOut>;
   allow_lds(kzi, zlds);
+  profiler().begin(s);
   hipLaunchKernelGGL(kzi, dim3(zblocks), dim3(kZBT), zlds, s, g_, pz_, twz_, P, spec_, out, r1,
                      reinterpret_cast<const T*>(in), st, partials);
   MTL_LAUNCH_CHECK();
+  profiler().end(MODE == kModeMatvec ? kKZInverse : residual_id(kKZInverse), s);
 }
 
 template <typename T>

@@ -115,8 +115,13 @@ class SmallStrainFFT:
     def run(self, material, orientation_label="orientation", phase_label=None,
             output_variables=None, output_times=None, global_tolerance=1.0e-3,
             strain_correction_tolerance=1.0e-2, linear_solver_tolerance=1.0e-5,
-            postprocessor=None):
-        """small_strain_fft.py:60-264."""
+            postprocessor=None, increment_callback=None):
+        """small_strain_fft.py:60-264.
+
+        ``increment_callback(info)`` (an extension) is called after every
+        converged increment with a dict of the increment's counters; a true
+        return value ends the run there.
+        """
         self._sizes = material.sizes
         self._dimensions = material.dimensions
         self._num_points = material.num_points
@@ -195,6 +200,13 @@ class SmallStrainFFT:
                         time_idx += 1
                         next_output_time = output_times[time_idx]
                     time_increment = new_time_increment
+                    if increment_callback is not None and increment_callback({
+                            "step": time_step_id, "time": time,
+                            "krylov_iterations": int(rep.krylov_iterations),
+                            "newton_iterations": int(rep.newton_iterations),
+                            "constitutive_iterations": int(rep.max_constitutive_iterations),
+                            "num_points": self._num_points}):
+                        break
                 else:
                     time_increment = time_increment / 2.0
                     native.rollback()

@@ -57,6 +57,9 @@ class Tensor:
     _rank = 0  # number of trailing component axes
 
     def __init__(self, components, dims=None):
+        if isinstance(components, Tensor):
+            dims = components.dims_str if dims is None else dims
+            components = components.components
         c = np.asarray(components, dtype=np.float64)
         nlead = c.ndim - self._rank
         if nlead < 0:

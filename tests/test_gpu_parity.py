@@ -49,3 +49,93 @@ def test_projection_matches_oracle(dims, precision):
         scale = np.abs(ref).max()
         tol = 1e-13 if precision == "fp64" else 2e-6
         assert np.abs(got - ref).max() <= tol * scale, (dims, precision, np.abs(got - ref).max() / scale)
+
+
+def _pore_case():
+    from oracle import constitutive as C
+    from tests import fixtures as F
+
+    rot, pore, _ = F.voronoi_8()
+    idx = [np.nonzero(~pore)[0], np.nonzero(pore)[0]]
+    mp = C.Multiphase([C.Elastic(F.TI_STIFFNESS), C.Elastic(F.TI_STIFFNESS * 0)], idx, 512)
+    return rot, pore, mp
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_minres_matches_oracle(precision):
+    from materialite_amd._native import NativeSolver
+    from oracle.minres import minres
+    from oracle.projection import apply_projection
+    from tests import fixtures as F
+
+    print(f"minres {precision}", flush=True)
+    rot, pore, mp = _pore_case()
+    tangent = mp.initialize(rot)
+    mask = np.array([1, 1, 0, 1, 1, 1.0])
+    sig = np.einsum("pmn,n->pm", tangent, [0, 0, 5e-5, 0, 0, 0])
+    b = apply_projection(-sig, (8, 8, 8), stress_mask=mask)
+
+    def lhs(x):
+        return apply_projection(np.einsum("pmn,pn->pm", tangent, x.reshape(512, 6)), (8, 8, 8),
+                                stress_mask=mask).ravel()
+
+    rtol = 1e-7 if precision == "fp64" else 1e-5
+    x_ref, itn_ref, _ = minres(lhs, b.ravel(), rtol=rtol, maxiter=5 * 6 * 512)
+    sol = NativeSolver((8, 8, 8), precision)
+    sol.add_elastic(F.TI_STIFFNESS)
+    sol.add_elastic(F.TI_STIFFNESS * 0)
+    sol.set_microstructure(rot, pore.astype(np.int32))
+    sol.initialize()
+    bd = _dev(_soa(b))
+    xd = torch.empty_like(bd)
+    itn = sol.minres_dev(bd.data_ptr(), xd.data_ptr(), mask, rtol)
+    x = xd.cpu().numpy().T.ravel()
+    sol.close()
+    scale = np.abs(x_ref).max()
+    if precision == "fp64":
+        assert abs(itn - itn_ref) <= 1, (itn, itn_ref)
+        assert np.abs(x - x_ref).max() <= 1e-9 * scale
+    else:
+        # an FP32 Krylov solve to rtol 1e-5 against the FP64 one to 1e-5
+        assert itn <= 2 * itn_ref + 5, (itn, itn_ref)
+        assert np.abs(x - x_ref).max() <= 1e-3 * scale
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_evp_update_matches_oracle(precision):
+    from materialite_amd._native import NativeSolver
+    from oracle import tensors as T
+    from tests import fixtures as F
+
+    print(f"evp update {precision}", flush=True)
+    rot, _, _ = F.voronoi_8()
+    props = {"tau_1": 100.0, "theta_0": 20000.0, "theta_1": 10.0}
+    model = F.evp_model("voce", props)
+    tangent = model.initialize(rot)
+    rng = np.random.default_rng(5)
+    strain = np.tile([-0.0012, -0.0012, 0.004, 0, 0, 0], (512, 1)) + 2e-4 * rng.normal(size=(512, 6))
+    guess = np.einsum("pmn,pn->pm", tangent, strain) * 0.5
+    stress_ref, tangent_ref, it_ref = model.calculate_stress_and_tangent(strain, guess, 1e-3)
+    assert it_ref
+    sol = NativeSolver((8, 8, 8), precision)
+    schmid = T.schmid_mandel(*T.octahedral_slip_systems())
+    sol.add_evp(F.TI_STIFFNESS, schmid, 1.0, 10.0, 300.0, 2,
+                [props["theta_0"], props["theta_1"], props["tau_1"]])
+    sol.set_microstructure(rot, np.zeros(512, np.int32))
+    sol.initialize()
+    sd, gd = _dev(_soa(strain)), _dev(_soa(guess))
+    out = torch.empty_like(sd)
+    it = sol.constitutive_dev(sd.data_ptr(), gd.data_ptr(), 1e-3, out.data_ptr())
+    stress = out.cpu().numpy().T
+    tan = sol.get_field("tangent").reshape(512, 6, 6)
+    g = sol.get_field("slip_resistances")
+    sol.close()
+    serr = np.abs(stress - stress_ref).max() / np.abs(stress_ref).max()
+    terr = np.abs(tan - tangent_ref).max() / np.abs(tangent_ref).max()
+    print(f"iterations {it} (oracle {it_ref}), stress err {serr:.3e}, tangent err {terr:.3e}",
+          flush=True)
+    assert it == it_ref, (it, it_ref)
+    assert np.abs(stress - stress_ref).max() <= 1e-10 * np.abs(stress_ref).max()
+    ttol = 1e-10 if precision == "fp64" else 1e-6
+    assert np.abs(tan - tangent_ref).max() <= ttol * np.abs(tangent_ref).max()
+    assert np.abs(g - model.g).max() <= 1e-10 * np.abs(model.g).max()
