@@ -11,7 +11,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmaterialite_amd.so")
+# MATERIALITE_AMD_LIB selects another in-tree build of the same library
+LIB_PATH = os.environ.get("MATERIALITE_AMD_LIB", os.path.join(HERE, "libmaterialite_amd.so"))
 
 MTL_FP64 = 0
 MTL_MIXED = 1

@@ -165,6 +165,17 @@ struct EvpParams {
   double hp[4];
 };
 
+// The model parameters (Schmid tensors, compliance: ~100 doubles) are staged in
+// LDS rather than passed by value: as kernel arguments they are held in SGPRs
+// and spill heavily, which produced wrong results in this kernel.
+template <typename P>
+__device__ __forceinline__ void load_params(P* dst, const P* src) {
+  static_assert(sizeof(P) % sizeof(int) == 0, "parameter block must be int-sized");
+  const int* s = reinterpret_cast<const int*>(src);
+  int* d = reinterpret_cast<int*>(dst);
+  for (int k = threadIdx.x; k < int(sizeof(P) / sizeof(int)); k += blockDim.x) d[k] = s[k];
+}
+
 struct EvpControl {
   int count[kEvpMaxIterations + 4];  // points converging in iteration it
   long long total;                   // points converged after the last iteration
@@ -395,11 +406,13 @@ __device__ __forceinline__ void load_point(const EvpBuffers& B, int n, int i, in
 // (elastic_viscoplastic.py:116-158).  A point's state after iteration j lives
 // in ping-pong buffer j&1; last[i] records the last iteration it ran.
 template <int NS>
-__global__ void __launch_bounds__(kCBT) k_evp_iter(int it, int n, EvpParams<NS> q, EvpBuffers B,
-                                                    EvpControl* ctrl) {
+__global__ void __launch_bounds__(kCBT) k_evp_iter(int it, int n, const EvpParams<NS>* qg,
+                                                    EvpBuffers B, EvpControl* ctrl) {
   if (ctrl->done) return;
   const int masked = ctrl->masked;
+  __shared__ EvpParams<NS> q;
   __shared__ int cnt;
+  load_params(&q, qg);
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   const int i = blockIdx.x * kCBT + threadIdx.x;
@@ -449,11 +462,16 @@ __global__ void k_evp_control(int it, int n, EvpControl* ctrl) {
 // from (bitwise the same stress and slip resistances as that iteration wrote).
 template <int NS, typename K>
 __global__ void __launch_bounds__(kCBT) k_evp_outputs(int n, const int* idx, long long npts,
-                                                       const double* rot9, EvpParams<NS> q,
-                                                       EvpBuffers B, const EvpControl* ctrl,
-                                                       double* sig_out, K* tangent) {
+                                                       const double* rot9,
+                                                       const EvpParams<NS>* qg, EvpBuffers B,
+                                                       const EvpControl* ctrl, double* sig_out,
+                                                       K* tangent) {
+  if (ctrl->fail || !ctrl->done) return;
+  __shared__ EvpParams<NS> q;
+  load_params(&q, qg);
+  __syncthreads();
   const int i = blockIdx.x * kCBT + threadIdx.x;
-  if (i >= n || ctrl->fail || !ctrl->done) return;
+  if (i >= n) return;
   const int src = (B.last[i] - 1) & 1;
   double sig[6], g[NS], eps[6], ep_old[6], g_old[NS], acc_old;
   double sig_new[6], g_new[NS], ep[6], dgam[NS], Sinv[21], diff;
@@ -631,6 +649,8 @@ class EvpModel final : public Model<K> {
     alloc(gam_tr_, NS * n);
     alloc(acc_tr_, n);
     MTL_HIP(hipMalloc(&ctrl_, sizeof(EvpControl)));
+    MTL_HIP(hipMalloc(&qd_, sizeof(EvpParams<NS>)));
+    MTL_HIP(hipHostMalloc(&qh_, sizeof(EvpParams<NS>), hipHostMallocDefault));
     MTL_HIP(hipHostMalloc(&hctrl_, sizeof(EvpControl), hipHostMallocDefault));
   }
   ~EvpModel() override {
@@ -641,6 +661,8 @@ class EvpModel final : public Model<K> {
     if (B_.conv) (void)hipFree(B_.conv);
     if (B_.last) (void)hipFree(B_.last);
     if (ctrl_) (void)hipFree(ctrl_);
+    if (qd_) (void)hipFree(qd_);
+    if (qh_) (void)hipHostFree(qh_);
     if (hctrl_) (void)hipHostFree(hctrl_);
     (void)hipFree(this->idx_);
   }
@@ -667,6 +689,8 @@ class EvpModel final : public Model<K> {
     const int n = this->n_;
     if (n == 0) return 1;
     q_.dt = dt;
+    *qh_ = q_;
+    MTL_HIP(hipMemcpyAsync(qd_, qh_, sizeof(EvpParams<NS>), hipMemcpyHostToDevice, s));
     const int blocks = ceil_div(n, kCBT);
     hipLaunchKernelGGL(k_evp_setup<NS>, dim3(blocks), dim3(kCBT), 0, s, n, this->idx_, this->npts_,
                        rot9, eps, guess, g_tr_, B_);
@@ -681,7 +705,7 @@ class EvpModel final : public Model<K> {
       for (int k = 0; k < chunk && it < kEvpMaxIterations; ++k) {
         ++it;
         profiler().begin(s);
-        hipLaunchKernelGGL(k_evp_iter<NS>, dim3(blocks), dim3(kCBT), 0, s, it, n, q_, B_, ctrl_);
+        hipLaunchKernelGGL(k_evp_iter<NS>, dim3(blocks), dim3(kCBT), 0, s, it, n, qd_, B_, ctrl_);
         MTL_LAUNCH_CHECK();
         profiler().end(kKEvpIter, s);
         hipLaunchKernelGGL(k_evp_control, dim3(1), dim3(64), 0, s, it, n, ctrl_);
@@ -702,7 +726,7 @@ class EvpModel final : public Model<K> {
     if (hctrl_->fail || !hctrl_->done) return 0;
     profiler().begin(s);
     hipLaunchKernelGGL((k_evp_outputs<NS, K>), dim3(blocks), dim3(kCBT), 0, s, n, this->idx_,
-                       this->npts_, rot9, q_, B_, ctrl_, sig, tangent);
+                       this->npts_, rot9, qd_, B_, ctrl_, sig, tangent);
     MTL_LAUNCH_CHECK();
     profiler().end(kKEvpFinish, s);
     // adopt the call's outputs as the trial state (elastic_viscoplastic.py:175-182)
@@ -768,6 +792,8 @@ class EvpModel final : public Model<K> {
   double *ep_tr_ = nullptr, *g_tr_ = nullptr, *gam_tr_ = nullptr, *acc_tr_ = nullptr;
   EvpControl* ctrl_ = nullptr;
   EvpControl* hctrl_ = nullptr;
+  EvpParams<NS>* qd_ = nullptr;  // device copy read by the kernels
+  EvpParams<NS>* qh_ = nullptr;  // pinned staging copy
 };
 
 }  // namespace
