@@ -98,4 +98,15 @@ __device__ __forceinline__ double block_max(double v, double* red) {
 
 inline int ceil_div(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
 
+// Copy a small parameter block from device memory into LDS with all threads of
+// the workgroup (the caller issues the barrier).  Large uniform parameter sets
+// passed by value live in SGPRs and spill; reading them from LDS avoids that.
+template <typename P>
+__device__ __forceinline__ void stage_to_lds(P* dst, const P* src) {
+  static_assert(sizeof(P) % sizeof(int) == 0, "parameter block must be int-sized");
+  const int* s = reinterpret_cast<const int*>(src);
+  int* d = reinterpret_cast<int*>(dst);
+  for (int k = threadIdx.x; k < int(sizeof(P) / sizeof(int)); k += blockDim.x) d[k] = s[k];
+}
+
 }  // namespace mtl

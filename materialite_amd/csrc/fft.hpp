@@ -1,15 +1,19 @@
 // Workgroup-level mixed-radix Stockham FFT over lines held in LDS.
 //
-// A workgroup of BT threads transforms `nlines` lines of length L stored
-// contiguously in LDS (line i at buf[i*L .. i*L+L)).  Each radix-R pass reads
-// the R inputs of its butterflies, applies twiddles w^(t*k) (k = j mod Ns),
+// A batch of M lines of length L is stored element-major: element e of line m
+// at buf[e * S + m] (row stride S >= M; an odd S spreads strided accesses
+// over the LDS banks).  Each radix-R pass reads the R inputs of a butterfly
+// (line m, index j) from one buffer, applies twiddles w^(t*k) (k = j mod Ns),
 // performs an R-point DFT and writes the outputs to their autosorted Stockham
-// positions (j/Ns)*Ns*R + k + u*Ns.  Outputs are staged in registers across a
-// barrier, so the pass runs in place: every thread holds at most PTS complex
-// values, and the caller guarantees nlines * L <= PTS * BT.
+// positions (j/Ns)*Ns*R + k + u*Ns of the other buffer (ping-pong, one barrier
+// per pass, no register staging).  Butterflies are numbered line-fastest, so
+// the lanes of a wavefront touch runs of consecutive LDS words in every pass.
 //
 // Twiddles come from a per-axis table tw[m] = exp(-2 pi i m / L) copied into
-// LDS; the inverse transform uses their conjugates (unnormalised).
+// LDS; the inverse transform uses their conjugates (unnormalised).  The odd
+// radices read their R-th roots of unity from the same table (tw[k L/R]):
+// held as literals, their FP64 constants would occupy scalar registers in every
+// inlined pass and push the kernels into scalar-register spills.
 #pragma once
 
 #include "common.hpp"
@@ -31,7 +35,7 @@ struct Dft;
 
 template <typename T>
 struct Dft<T, 2> {
-  __device__ __forceinline__ static void run(cplx<T>* v, bool) {
+  __device__ __forceinline__ static void run(cplx<T>* v, bool, const cplx<T>*, int) {
     cplx<T> a = v[0], b = v[1];
     v[0] = cadd(a, b);
     v[1] = csub(a, b);
@@ -54,11 +58,14 @@ struct Dft<T, 4> {
     v[1] = cadd(a1, a3);
     v[3] = csub(a1, a3);
   }
+  __device__ __forceinline__ static void run(cplx<T>* v, bool inv, const cplx<T>*, int) {
+    run(v, inv);
+  }
 };
 
 template <typename T>
 struct Dft<T, 8> {
-  __device__ __forceinline__ static void run(cplx<T>* v, bool inv) {
+  __device__ __forceinline__ static void run(cplx<T>* v, bool inv, const cplx<T>*, int) {
     const T h = T(0.70710678118654752440084436210484903928);
     cplx<T> b[4], c[4];
 #pragma unroll
@@ -86,8 +93,8 @@ struct Dft<T, 8> {
 
 template <typename T>
 struct Dft<T, 3> {
-  __device__ __forceinline__ static void run(cplx<T>* v, bool inv) {
-    const T s = T(0.86602540378443864676372317075293618347);
+  __device__ __forceinline__ static void run(cplx<T>* v, bool inv, const cplx<T>* rt, int q) {
+    const T s = -rt[q].y;  // sin(2 pi/3)
     cplx<T> t1 = cadd(v[1], v[2]);
     cplx<T> t2 = {v[0].x - T(0.5) * t1.x, v[0].y - T(0.5) * t1.y};
     cplx<T> d = csub(v[1], v[2]);
@@ -100,11 +107,9 @@ struct Dft<T, 3> {
 
 template <typename T>
 struct Dft<T, 5> {
-  __device__ __forceinline__ static void run(cplx<T>* v, bool inv) {
-    const T c1 = T(0.30901699437494742410229341718281905886);
-    const T c2 = T(-0.80901699437494742410229341718281905886);
-    const T s1 = T(0.95105651629515357211643933337938214340);
-    const T s2 = T(0.58778525229247312916870595463907276860);
+  __device__ __forceinline__ static void run(cplx<T>* v, bool inv, const cplx<T>* rt, int q) {
+    const cplx<T> w1 = rt[q], w2 = rt[2 * q];  // exp(-2 pi i k/5)
+    const T c1 = w1.x, c2 = w2.x, s1 = -w1.y, s2 = -w2.y;
     cplx<T> a1 = cadd(v[1], v[4]), b1 = csub(v[1], v[4]);
     cplx<T> a2 = cadd(v[2], v[3]), b2 = csub(v[2], v[3]);
     cplx<T> m1 = {v[0].x + c1 * a1.x + c2 * a2.x, v[0].y + c1 * a1.y + c2 * a2.y};
@@ -122,98 +127,93 @@ struct Dft<T, 5> {
 
 template <typename T>
 struct Dft<T, 7> {
-  __device__ __forceinline__ static void run(cplx<T>* v, bool inv) {
-    // direct 7-point DFT with exact-to-T constant roots
-    const T cs[7] = {T(1), T(0.62348980185873353053), T(-0.22252093395631440429),
-                     T(-0.90096886790241912624), T(-0.90096886790241912624),
-                     T(-0.22252093395631440429), T(0.62348980185873353053)};
-    const T sn[7] = {T(0), T(0.78183148246802980871), T(0.97492791218182360702),
-                     T(0.43388373911755812048), T(-0.43388373911755812048),
-                     T(-0.97492791218182360702), T(-0.78183148246802980871)};
-    cplx<T> in[7];
-#pragma unroll
-    for (int t = 0; t < 7; ++t) in[t] = v[t];
-#pragma unroll
-    for (int u = 0; u < 7; ++u) {
-      cplx<T> acc = {T(0), T(0)};
-#pragma unroll
-      for (int t = 0; t < 7; ++t) {
-        const int m = (t * u) % 7;
-        const T c = cs[m], s = inv ? sn[m] : -sn[m];
-        acc.x += in[t].x * c - in[t].y * s;
-        acc.y += in[t].x * s + in[t].y * c;
-      }
-      v[u] = acc;
-    }
+  __device__ __forceinline__ static void run(cplx<T>* v, bool inv, const cplx<T>* rt, int q) {
+    // symmetric form: with a_k = v_k + v_{7-k}, b_k = v_k - v_{7-k},
+    //   y_u, y_{7-u} = v_0 + sum_k cos(2 pi uk/7) a_k  -+ i sum_k sin(2 pi uk/7) b_k
+    const cplx<T> w1 = rt[q], w2 = rt[2 * q], w3 = rt[3 * q];  // exp(-2 pi i k/7)
+    const T c1 = w1.x, c2 = w2.x, c3 = w3.x;
+    const T s1 = -w1.y, s2 = -w2.y, s3 = -w3.y;
+    const cplx<T> a1 = cadd(v[1], v[6]), b1 = csub(v[1], v[6]);
+    const cplx<T> a2 = cadd(v[2], v[5]), b2 = csub(v[2], v[5]);
+    const cplx<T> a3 = cadd(v[3], v[4]), b3 = csub(v[3], v[4]);
+    const cplx<T> x0 = v[0];
+    const cplx<T> m1 = {x0.x + c1 * a1.x + c2 * a2.x + c3 * a3.x, x0.y + c1 * a1.y + c2 * a2.y + c3 * a3.y};
+    const cplx<T> m2 = {x0.x + c2 * a1.x + c3 * a2.x + c1 * a3.x, x0.y + c2 * a1.y + c3 * a2.y + c1 * a3.y};
+    const cplx<T> m3 = {x0.x + c3 * a1.x + c1 * a2.x + c2 * a3.x, x0.y + c3 * a1.y + c1 * a2.y + c2 * a3.y};
+    const cplx<T> n1 = {s1 * b1.x + s2 * b2.x + s3 * b3.x, s1 * b1.y + s2 * b2.y + s3 * b3.y};
+    const cplx<T> n2 = {s2 * b1.x - s3 * b2.x - s1 * b3.x, s2 * b1.y - s3 * b2.y - s1 * b3.y};
+    const cplx<T> n3 = {s3 * b1.x - s1 * b2.x + s2 * b3.x, s3 * b1.y - s1 * b2.y + s2 * b3.y};
+    v[0] = {x0.x + a1.x + a2.x + a3.x, x0.y + a1.y + a2.y + a3.y};
+    const cplx<T> r1 = rot_i(n1, inv), r2 = rot_i(n2, inv), r3 = rot_i(n3, inv);
+    v[1] = cadd(m1, r1);
+    v[6] = csub(m1, r1);
+    v[2] = cadd(m2, r2);
+    v[5] = csub(m2, r2);
+    v[3] = cadd(m3, r3);
+    v[4] = csub(m3, r3);
   }
 };
 
-// --------------------------------------------------------------- one pass
-template <typename T, int BT, int PTS, int R>
-__device__ __forceinline__ void fft_pass(cplx<T>* buf, int nlines, int L, int Ns,
-                                         const cplx<T>* tw, bool inv) {
-  constexpr int KMAX = (PTS + R - 1) / R;
-  const int per_line = L / R;
-  const int total = nlines * per_line;
-  const int tstride = L / (Ns * R);
-  cplx<T> st[KMAX * R];
+
+// --------------------------------------------------------------- one butterfly
+template <typename T, int R>
+__device__ __forceinline__ void butterfly(const cplx<T>* __restrict__ src, cplx<T>* __restrict__ dst,
+                                          int S, int j, int m, int kk, int jq, int per_line,
+                                          int Ns, int tstride, const cplx<T>* tw, bool inv) {
+  cplx<T> v[R];
 #pragma unroll
-  for (int k = 0; k < KMAX; ++k) {
-    const int jj = threadIdx.x + k * BT;
-    if (jj < total) {
-      const int line = jj / per_line;
-      const int j = jj - line * per_line;
-      const int kk = j % Ns;
-      const cplx<T>* a = buf + line * L;
-      cplx<T> v[R];
-#pragma unroll
-      for (int t = 0; t < R; ++t) {
-        cplx<T> x = a[j + t * per_line];
-        if (t > 0 && kk > 0) {
-          cplx<T> w = tw[t * kk * tstride];
-          if (inv) w.y = -w.y;
-          x = cmul(x, w);
-        }
-        v[t] = x;
-      }
-      Dft<T, R>::run(v, inv);
-#pragma unroll
-      for (int u = 0; u < R; ++u) st[k * R + u] = v[u];
+  for (int t = 0; t < R; ++t) {
+    cplx<T> x = src[(j + t * per_line) * S + m];
+    if (t > 0 && kk > 0) {
+      cplx<T> w = tw[t * kk * tstride];
+      if (inv) w.y = -w.y;
+      x = cmul(x, w);
     }
+    v[t] = x;
   }
-  __syncthreads();
+  Dft<T, R>::run(v, inv, tw, per_line);
+  const int idx = jq * Ns * R + kk;
 #pragma unroll
-  for (int k = 0; k < KMAX; ++k) {
-    const int jj = threadIdx.x + k * BT;
-    if (jj < total) {
-      const int line = jj / per_line;
-      const int j = jj - line * per_line;
-      const int kk = j % Ns;
-      cplx<T>* a = buf + line * L;
-      const int idx = (j / Ns) * Ns * R + kk;
-#pragma unroll
-      for (int u = 0; u < R; ++u) a[idx + u * Ns] = st[k * R + u];
-    }
-  }
-  __syncthreads();
+  for (int u = 0; u < R; ++u) dst[(idx + u * Ns) * S + m] = v[u];
 }
 
-// Transform all lines in LDS.  Ends with a barrier.
-template <typename T, int BT, int PTS>
-__device__ void block_fft(cplx<T>* buf, int nlines, const AxisPlan& pl, const cplx<T>* tw,
-                          bool inv) {
-  for (int p = 0; p < pl.npass; ++p) {
-    const int R = pl.radix[p], Ns = pl.ns[p];
-    switch (R) {
-      case 8: fft_pass<T, BT, PTS, 8>(buf, nlines, pl.L, Ns, tw, inv); break;
-      case 4: fft_pass<T, BT, PTS, 4>(buf, nlines, pl.L, Ns, tw, inv); break;
-      case 2: fft_pass<T, BT, PTS, 2>(buf, nlines, pl.L, Ns, tw, inv); break;
-      case 3: fft_pass<T, BT, PTS, 3>(buf, nlines, pl.L, Ns, tw, inv); break;
-      case 5: fft_pass<T, BT, PTS, 5>(buf, nlines, pl.L, Ns, tw, inv); break;
-      case 7: fft_pass<T, BT, PTS, 7>(buf, nlines, pl.L, Ns, tw, inv); break;
-      default: break;  // rejected on the host when the plan is built
+// Transform M interleaved lines held in `a` (row stride S >= M complex values),
+// using `b` as the second buffer.  Returns the buffer that holds the result.
+// Ends with a barrier.  The radix switch sits inside the butterfly loop so that
+// the per-pass index arithmetic is shared by all radices.
+template <typename T, int BT>
+__device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int S,
+                                              const AxisPlan* __restrict__ pl, const cplx<T>* tw,
+                                              bool inv) {
+  const int npass = pl->npass;
+  const int L = pl->L;
+  for (int p = 0; p < npass; ++p) {
+    const int R = pl->radix[p];
+    const int Ns = pl->ns[p];
+    const int per_line = L / R;
+    const int total = M * per_line;
+    const int tstride = per_line / Ns;
+    for (int bf = threadIdx.x; bf < total; bf += BT) {
+      const int j = bf / M;
+      const int m = bf - j * M;
+      const int jq = j / Ns;
+      const int kk = j - jq * Ns;
+      switch (R) {
+        case 8: butterfly<T, 8>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
+        case 4: butterfly<T, 4>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
+        case 2: butterfly<T, 2>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
+        case 3: butterfly<T, 3>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
+        case 5: butterfly<T, 5>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
+        case 7: butterfly<T, 7>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
+        default: break;  // rejected on the host when the plan is built
+      }
     }
+    __syncthreads();
+    cplx<T>* t = a;
+    a = b;
+    b = t;
   }
+  return a;
 }
 
 // Host: factor L into radices 8,4,2,3,5,7 (largest first). Throws if impossible.

@@ -11,6 +11,7 @@
 // t_j = tan(pi q_j) (rotated finite-difference frequencies of
 // small_strain_fft.py:301-328); zero on even-N Nyquist planes (:354-359) and
 // diag(mask) at k = 0 (:363-367).
+#include <algorithm>
 #include <cmath>
 #include <mutex>
 #include <unordered_map>
@@ -23,9 +24,16 @@ namespace mtl {
 
 namespace {
 
-constexpr int kZBT = 256, kZPTS = 16;  // z passes: <= 4096 points per workgroup
-constexpr int kYBT = 256, kYPTS = 16;  // y pass:   <= 4096 points per workgroup
-constexpr int kXBT = 512, kXPTS = 16;  // x pass:   <= 8192 points per workgroup
+// threads per workgroup of each pass
+constexpr int kZBT = 256, kYBT = 256, kXBT = 512;
+// LDS bytes for the two ping-pong line buffers of one workgroup: about 64 KB
+// lets two or more workgroups share a CU's 160 KB
+constexpr size_t kZBufBytes = 64 * 1024, kYBufBytes = 64 * 1024, kXBufBytes = 96 * 1024;
+constexpr size_t kMaxLds = 160 * 1024;
+
+// LDS row stride of the z passes: 3 interleaved lines (paired components) per
+// grid line, rounded up to odd
+__host__ __device__ inline int zrow_stride(int P) { return (3 * P) | 1; }
 
 enum { kModeProject = 0, kModeMatvec = 1 };
 
@@ -36,15 +44,17 @@ __device__ __forceinline__ void load_twiddles(cplx<T>* dst, const cplx<T>* src, 
 
 // ----------------------------------------------------------------- z forward
 template <typename T, int MODE, typename TIn>
-__global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, AxisPlan pz, const cplx<T>* twz,
+__global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan_g, const cplx<T>* twz,
                                                      int P, const TIn* in, const T* tangent,
                                                      Vec6 a, double sgn,
                                                      const MinresState* st, cplx<T>* spec) {
   if (MODE == kModeMatvec && st->done) return;
   extern __shared__ unsigned char smem_raw[];
+  const AxisPlan* plan = plan_g;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + g.nz;
   const int nz = g.nz, nzc = g.nzc;
+  const int M = 3 * P, S = zrow_stride(P);
   const long long nlines = (long long)g.nx * g.ny;
   const long long line0 = (long long)blockIdx.x * P;
   load_twiddles(tw, twz, nz);
@@ -74,10 +84,10 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, AxisPlan pz, const c
       }
     }
 #pragma unroll
-    for (int q = 0; q < 3; ++q) buf[(li * 3 + q) * nz + z] = {tau[2 * q], tau[2 * q + 1]};
+    for (int q = 0; q < 3; ++q) buf[z * S + li * 3 + q] = {tau[2 * q], tau[2 * q + 1]};
   }
   __syncthreads();
-  block_fft<T, kZBT, kZPTS>(buf, 3 * P, pz, tw, false);
+  const cplx<T>* res = block_fft<T, kZBT>(buf, buf + S * nz, M, S, plan, tw, false);
 
   const T half = T(0.5);
   for (int i = threadIdx.x; i < P * 3 * nzc; i += kZBT) {
@@ -86,9 +96,9 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, AxisPlan pz, const c
     const int q = rem / nzc, kz = rem - q * nzc;
     const long long gl = line0 + li;
     if (gl >= nlines) continue;
-    const cplx<T>* line = buf + (li * 3 + q) * nz;
-    const cplx<T> zf = line[kz];
-    const cplx<T> zr = cconj(line[(nz - kz) % nz]);
+    const int m = li * 3 + q;
+    const cplx<T> zf = res[kz * S + m];
+    const cplx<T> zr = cconj(res[((nz - kz) % nz) * S + m]);
     const cplx<T> xa = {half * (zf.x + zr.x), half * (zf.y + zr.y)};
     const cplx<T> xb = {half * (zf.y - zr.y), -half * (zf.x - zr.x)};
     spec[(2 * q) * g.nspec + gl * nzc + kz] = xa;
@@ -98,11 +108,12 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, AxisPlan pz, const c
 
 // ----------------------------------------------------------------- y pass
 template <typename T>
-__global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, AxisPlan py, const cplx<T>* twy, int TY,
+__global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g, const cplx<T>* twy, int TY,
                                                   int ntiles, bool inv,
                                                   const MinresState* st, cplx<T>* spec) {
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
+  const AxisPlan* plan = plan_g;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + g.ny;
   const int b = xcd_logical_block(blockIdx.x, gridDim.x);
@@ -115,13 +126,13 @@ __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, AxisPlan py, const cplx
   load_twiddles(tw, twy, ny);
   for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
     const int y = i / TY, t = i - y * TY;
-    buf[t * ny + y] = t < width ? base[(long long)y * g.nzc + t] : cplx<T>{T(0), T(0)};
+    buf[y * TY + t] = t < width ? base[(long long)y * g.nzc + t] : cplx<T>{T(0), T(0)};
   }
   __syncthreads();
-  block_fft<T, kYBT, kYPTS>(buf, TY, py, tw, inv);
+  const cplx<T>* res = block_fft<T, kYBT>(buf, buf + TY * ny, TY, TY, plan, tw, inv);
   for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
     const int y = i / TY, t = i - y * TY;
-    if (t < width) base[(long long)y * g.nzc + t] = buf[t * ny + y];
+    if (t < width) base[(long long)y * g.nzc + t] = res[y * TY + t];
   }
 }
 
@@ -159,12 +170,13 @@ __device__ __forceinline__ double rnorm3<double>(double a, double b, double c) {
 }
 
 template <typename T>
-__global__ void __launch_bounds__(kXBT) k_x_ghat(Grid g, AxisPlan px, const cplx<T>* twx, int TX,
+__global__ void __launch_bounds__(kXBT) k_x_ghat(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
-                                                  const T* tanz, Vec6 mask,
+                                                  const T* tanz, int maskbits,
                                                   const MinresState* st, cplx<T>* spec) {
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
+  const AxisPlan* plan = plan_g;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + g.nx;
   const int b = xcd_logical_block(blockIdx.x, gridDim.x);
@@ -181,59 +193,62 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat(Grid g, AxisPlan px, const cplx
     const int t = i % TX;
     const int cxp = i / TX;  // c*nx + x
     const int c = cxp / nx, x = cxp - c * nx;
-    buf[(c * TX + t) * nx + x] =
+    buf[x * (6 * TX) + c * TX + t] =
         t < width ? base[c * g.nspec + x * xstride + t] : cplx<T>{T(0), T(0)};
   }
   __syncthreads();
-  block_fft<T, kXBT, kXPTS>(buf, 6 * TX, px, tw, false);
-
   const int nyqx = (nx % 2 == 0) ? nx / 2 : -1;
   const int nyqy = (g.ny % 2 == 0) ? g.ny / 2 : -1;
   const int nyqz = (g.nz % 2 == 0) ? g.nz / 2 : -1;  // never stored, kept for clarity
   const T ty = tany[y];
+  cplx<T>* buf2 = buf + 6 * TX * nx;
+  cplx<T>* res = block_fft<T, kXBT>(buf, buf2, 6 * TX, 6 * TX, plan, tw, false);
   for (int i = threadIdx.x; i < nx * TX; i += kXBT) {
     const int t = i % TX, kx = i / TX;
     const int kz = kz0 + t;
     if (t >= width) continue;
     cplx<T> X[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) X[c] = buf[(c * TX + t) * nx + kx];
+    for (int c = 0; c < 6; ++c) X[c] = res[kx * (6 * TX) + c * TX + t];
     if (kx == nyqx || y == nyqy || kz == nyqz) {
 #pragma unroll
       for (int c = 0; c < 6; ++c) X[c] = {T(0), T(0)};
     } else if (kx == 0 && y == 0 && kz == 0) {
 #pragma unroll
-      for (int c = 0; c < 6; ++c) X[c] = mask.v[c] != 0.0 ? X[c] : cplx<T>{T(0), T(0)};
+      for (int c = 0; c < 6; ++c) X[c] = (maskbits >> c) & 1 ? X[c] : cplx<T>{T(0), T(0)};
     } else {
       const T tx = tanx[kx], tz = tanz[kz];
       const T inv = rnorm3<T>(tx, ty, tz);
       apply_ghat<T>(X, tx * inv, ty * inv, tz * inv);
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) buf[(c * TX + t) * nx + kx] = X[c];
+    for (int c = 0; c < 6; ++c) res[kx * (6 * TX) + c * TX + t] = X[c];
   }
   __syncthreads();
-  block_fft<T, kXBT, kXPTS>(buf, 6 * TX, px, tw, true);
+  const cplx<T>* out =
+      block_fft<T, kXBT>(res, res == buf ? buf2 : buf, 6 * TX, 6 * TX, plan, tw, true);
   for (int i = threadIdx.x; i < tot; i += kXBT) {
     const int t = i % TX;
     const int cxp = i / TX;
     const int c = cxp / nx, x = cxp - c * nx;
-    if (t < width) base[c * g.nspec + x * xstride + t] = buf[(c * TX + t) * nx + x];
+    if (t < width) base[c * g.nspec + x * xstride + t] = out[x * (6 * TX) + c * TX + t];
   }
 }
 
 // ----------------------------------------------------------------- z inverse
 template <typename T, int MODE, typename TOut>
-__global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, AxisPlan pz, const cplx<T>* twz,
+__global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan_g, const cplx<T>* twz,
                                                      int P, const cplx<T>* spec, TOut* out,
                                                      T* r1, const T* r2,
                                                      const MinresState* st, double* partials) {
   if (MODE == kModeMatvec && st->done) return;
   extern __shared__ unsigned char smem_raw[];
+  const AxisPlan* plan = plan_g;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + g.nz;
   __shared__ double red[kZBT / 64];
   const int nz = g.nz, nzc = g.nzc;
+  const int M = 3 * P, S = zrow_stride(P);
   const long long nlines = (long long)g.nx * g.ny;
   const long long line0 = (long long)blockIdx.x * P;
   load_twiddles(tw, twz, nz);
@@ -256,10 +271,10 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, AxisPlan pz, const c
         zv = {xa.x - xb.y, xa.y + xb.x};
       }
     }
-    buf[(li * 3 + q) * nz + k] = zv;
+    buf[k * S + li * 3 + q] = zv;
   }
   __syncthreads();
-  block_fft<T, kZBT, kZPTS>(buf, 3 * P, pz, tw, true);
+  const cplx<T>* res = block_fft<T, kZBT>(buf, buf + S * nz, M, S, plan, tw, true);
 
   const T invn = T(1.0 / double(g.n));
   double acc = 0.0;
@@ -275,7 +290,7 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, AxisPlan pz, const c
     const long long p = gl * nz + z;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const cplx<T> zv = buf[(li * 3 + q) * nz + z];
+      const cplx<T> zv = res[z * S + li * 3 + q];
       const T va = zv.x * invn, vb = zv.y * invn;
       if (MODE == kModeProject) {
         out[(2 * q) * g.n + p] = TOut(va);
@@ -325,6 +340,14 @@ U* upload(const std::vector<U>& v) {
   return d;
 }
 
+// bit c set when strain component c is controlled (mask != 0)
+int mask_bits(const Vec6& mask) {
+  int b = 0;
+  for (int c = 0; c < 6; ++c)
+    if (mask.v[c] != 0.0) b |= 1 << c;
+  return b;
+}
+
 // Raise a kernel's dynamic-LDS limit once to the largest size it is launched with.
 template <typename K>
 void allow_lds(K kernel, size_t bytes) {
@@ -352,19 +375,27 @@ Spectral<T>::Spectral(int nx, int ny, int nz) {
   px_ = make_axis_plan(nx);
   py_ = make_axis_plan(ny);
   pz_ = make_axis_plan(nz);
-  const int zcap = kZBT * kZPTS;
-  if (3 * nz > zcap) throw Error("nz too large for the z pass (max 1365)");
-  pz_lines_ = zcap / (3 * nz);
+  const size_t cb = sizeof(cplx<T>);
+  // z passes: P grid lines (3P interleaved complex lines) per workgroup
+  if ((size_t(nz) + 2 * size_t(zrow_stride(1)) * nz) * cb > kMaxLds)
+    throw Error("nz too large for the z pass");
+  pz_lines_ = int(std::max<size_t>(1, kZBufBytes / (2 * 3 * size_t(nz) * cb)));
+  while (pz_lines_ > 1 && 2 * size_t(zrow_stride(pz_lines_)) * nz * cb > kZBufBytes) --pz_lines_;
   const long long nlines = (long long)nx * ny;
   if (pz_lines_ > nlines) pz_lines_ = int(nlines);
-  if (ny > kYBT * kYPTS) throw Error("ny too large for the y pass");
-  ty_ = std::min({16, (kYBT * kYPTS) / ny, g_.nzc});
-  if (6 * nx > kXBT * kXPTS) throw Error("nx too large for the x pass");
-  tx_ = std::min({8, (kXBT * kXPTS) / (6 * nx), g_.nzc});
+  // y pass: ty_ kz columns per workgroup
+  if ((size_t(ny) + 2 * size_t(ny)) * cb > kMaxLds) throw Error("ny too large for the y pass");
+  ty_ = int(std::max<size_t>(1, std::min<size_t>({16, size_t(g_.nzc), kYBufBytes / (2 * ny * cb)})));
+  // x pass: tx_ kz columns of all 6 components per workgroup
+  if ((size_t(nx) + 2 * 6 * size_t(nx)) * cb > kMaxLds) throw Error("nx too large for the x pass");
+  tx_ = int(std::max<size_t>(1, std::min<size_t>({8, size_t(g_.nzc), kXBufBytes / (2 * 6 * nx * cb)})));
   MTL_HIP(hipMalloc(&spec_, 6 * g_.nspec * sizeof(cplx<T>)));
   twx_ = upload(twiddle_table<T>(nx));
   twy_ = upload(twiddle_table<T>(ny));
   twz_ = upload(twiddle_table<T>(nz));
+  pxd_ = upload(std::vector<AxisPlan>{px_});
+  pyd_ = upload(std::vector<AxisPlan>{py_});
+  pzd_ = upload(std::vector<AxisPlan>{pz_});
   tanx_ = upload(tan_table<T>(nx, nx));
   tany_ = upload(tan_table<T>(ny, ny));
   tanz_ = upload(tan_table<T>(nz, g_.nzc));
@@ -373,7 +404,7 @@ Spectral<T>::Spectral(int nx, int ny, int nz) {
 template <typename T>
 Spectral<T>::~Spectral() {
   for (void* p : {(void*)spec_, (void*)twx_, (void*)twy_, (void*)twz_, (void*)tanx_,
-                  (void*)tany_, (void*)tanz_})
+                  (void*)tany_, (void*)tanz_, (void*)pxd_, (void*)pyd_, (void*)pzd_})
     if (p) (void)hipFree(p);
 }
 
@@ -389,38 +420,38 @@ void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const V
                       hipStream_t s) {
   const int P = pz_lines_;
   const int zblocks = zinv_blocks();
-  const size_t zlds = (size_t(g_.nz) + size_t(3 * P) * g_.nz) * sizeof(cplx<T>);
+  const size_t zlds = (size_t(g_.nz) + 2 * size_t(zrow_stride(P)) * g_.nz) * sizeof(cplx<T>);
   auto kzf = k_z_forward<T, MODE, TIn>;
   allow_lds(kzf, zlds);
   profiler().begin(s);
-  hipLaunchKernelGGL(kzf, dim3(zblocks), dim3(kZBT), zlds, s, g_, pz_, twz_, P, in, tangent, a,
+  hipLaunchKernelGGL(kzf, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, in, tangent, a,
                      sgn, st, spec_);
   MTL_LAUNCH_CHECK();
   profiler().end(MODE == kModeMatvec ? kKZForward : residual_id(kKZForward), s);
 
   const MinresState* stp = MODE == kModeMatvec ? st : nullptr;
   const int ytiles = ceil_div(g_.nzc, ty_);
-  const size_t ylds = (size_t(g_.ny) + size_t(ty_) * g_.ny) * sizeof(cplx<T>);
+  const size_t ylds = (size_t(g_.ny) + 2 * size_t(ty_) * g_.ny) * sizeof(cplx<T>);
   auto ky = k_y_pass<T>;
   allow_lds(ky, ylds);
   profiler().begin(s);
-  hipLaunchKernelGGL(ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, py_, twy_, ty_, ytiles,
+  hipLaunchKernelGGL(ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles,
                      false, stp, spec_);
   MTL_LAUNCH_CHECK();
   profiler().end(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), s);
 
   const int xtiles = ceil_div(g_.nzc, tx_);
-  const size_t xlds = (size_t(g_.nx) + size_t(6 * tx_) * g_.nx) * sizeof(cplx<T>);
+  const size_t xlds = (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<T>);
   auto kx = k_x_ghat<T>;
   allow_lds(kx, xlds);
   profiler().begin(s);
-  hipLaunchKernelGGL(kx, dim3(g_.ny * xtiles), dim3(kXBT), xlds, s, g_, px_, twx_, tx_, xtiles,
-                     tanx_, tany_, tanz_, mask, stp, spec_);
+  hipLaunchKernelGGL(kx, dim3(g_.ny * xtiles), dim3(kXBT), xlds, s, g_, pxd_, twx_, tx_, xtiles,
+                     tanx_, tany_, tanz_, mask_bits(mask), stp, spec_);
   MTL_LAUNCH_CHECK();
   profiler().end(MODE == kModeMatvec ? kKXGhat : residual_id(kKXGhat), s);
 
   profiler().begin(s);
-  hipLaunchKernelGGL(ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, py_, twy_, ty_, ytiles,
+  hipLaunchKernelGGL(ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles,
                      true, stp, spec_);
   MTL_LAUNCH_CHECK();
   profiler().end(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), s);
@@ -428,7 +459,7 @@ void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const V
   auto kzi = k_z_inverse<T, MODE, TOut>;
   allow_lds(kzi, zlds);
   profiler().begin(s);
-  hipLaunchKernelGGL(kzi, dim3(zblocks), dim3(kZBT), zlds, s, g_, pz_, twz_, P, spec_, out, r1,
+  hipLaunchKernelGGL(kzi, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, spec_, out, r1,
                      reinterpret_cast<const T*>(in), st, partials);
   MTL_LAUNCH_CHECK();
   profiler().end(MODE == kModeMatvec ? kKZInverse : residual_id(kKZInverse), s);

@@ -47,6 +47,7 @@ class Spectral {
 
   Grid g_;
   AxisPlan px_, py_, pz_;
+  AxisPlan *pxd_ = nullptr, *pyd_ = nullptr, *pzd_ = nullptr;  // device copies
   int pz_lines_;  // z-lines per workgroup in the z passes
   int ty_;        // kz columns per workgroup in the y pass
   int tx_;        // kz columns per workgroup in the x pass
