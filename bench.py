@@ -257,6 +257,7 @@ def main():
             "mean_launch_us": mean_s * 1e6,
         },
         "kernels_ms": {k: round(v[0], 3) for k, v in prof.items()},
+        "kernels_launches": {k: int(v[1]) for k, v in prof.items()},
         "fp64_form": fp64,
         "cpu_baseline": cpu,
     }

@@ -15,9 +15,13 @@
 // Iteration order follows the reference exactly: every point iterates until at
 // least half of the model's points satisfy the convergence test (iteration
 // K*), after which only the unconverged points iterate, each independently, up
-// to 60 iterations in total.  The unmasked phase runs one kernel per iteration
-// with the state in HBM (ping-pong buffers and a device converged-count); the
-// masked phase and the outputs run in one kernel with the state in registers.
+// to 60 iterations in total.  Each iteration is one kernel over the state in
+// HBM (ping-pong buffers and a device converged-count).  The iterate's slip
+// resistances are not stored: every hardening law gives g_s as a function of
+// the committed g_old_s, the accumulated slip and the scalar slip increment
+// d = dt sum_s |rate_s| of the previous iteration, so one double per point
+// (d) replaces NS read and NS written values per iteration; recomputing g_s
+// from it repeats the same operations and is bitwise identical.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -187,9 +191,13 @@ struct EvpControl {
   int pad;
 };
 
+// |x|^ef; e = ef when ef is an integer in [0, 64] (INTPOW), then by squaring.
+// Separate instantiations keep the registers of the general pow() out of the
+// integer-exponent kernels.
+template <bool INTPOW>
 __device__ __forceinline__ double powi_abs(double x, int e, double ef) {
   x = fabs(x);
-  if (e < 0) return pow(x, ef);
+  if (!INTPOW) return pow(x, ef);
   double r = 1.0, b = x;
   while (e) {
     if (e & 1) r *= b;
@@ -199,34 +207,56 @@ __device__ __forceinline__ double powi_abs(double x, int e, double ef) {
   return r;
 }
 
-// Cholesky-based inverse of a packed SPD 6x6.
-__device__ __forceinline__ void spd_inverse6(const double S[21], double Sinv[21]) {
-  double L[6][6];
+// In-place Cholesky factor of a packed SPD 6x6: on return S holds the lower
+// factor, L[i][j] (i > j) at sym6(i, j), with 1 / L[j][j] on the diagonal.
+__device__ __forceinline__ void spd_factor6(double S[21]) {
 #pragma unroll
   for (int j = 0; j < 6; ++j) {
-    double d = S[sym6(j, j)];
+    double dg = S[sym6(j, j)];
 #pragma unroll
-    for (int k = 0; k < j; ++k) d -= L[j][k] * L[j][k];
-    const double ljj = sqrt(d);
-    L[j][j] = ljj;
-    const double il = 1.0 / ljj;
+    for (int k = 0; k < j; ++k) dg -= S[sym6(j, k)] * S[sym6(j, k)];
+    const double il = 1.0 / sqrt(dg);
+    S[sym6(j, j)] = il;
 #pragma unroll
     for (int i = j + 1; i < 6; ++i) {
       double v = S[sym6(i, j)];
 #pragma unroll
-      for (int k = 0; k < j; ++k) v -= L[i][k] * L[j][k];
-      L[i][j] = v * il;
+      for (int k = 0; k < j; ++k) v -= S[sym6(i, k)] * S[sym6(j, k)];
+      S[sym6(i, j)] = v * il;
     }
   }
+}
+
+// x = (L L^T)^-1 r for a factor from spd_factor6
+__device__ __forceinline__ void spd_solve6(const double L[21], const double r[6], double x[6]) {
+  double y[6];
+#pragma unroll
+  for (int i = 0; i < 6; ++i) {
+    double v = r[i];
+#pragma unroll
+    for (int k = 0; k < i; ++k) v -= L[sym6(i, k)] * y[k];
+    y[i] = v * L[sym6(i, i)];
+  }
+#pragma unroll
+  for (int i = 5; i >= 0; --i) {
+    double v = y[i];
+#pragma unroll
+    for (int k = i + 1; k < 6; ++k) v -= L[sym6(k, i)] * x[k];
+    x[i] = v * L[sym6(i, i)];
+  }
+}
+
+// (L L^T)^-1, packed, for a factor from spd_factor6
+__device__ __forceinline__ void spd_inverse6(const double L[21], double Sinv[21]) {
   double Li[6][6];
 #pragma unroll
   for (int i = 0; i < 6; ++i) {
-    Li[i][i] = 1.0 / L[i][i];
+    Li[i][i] = L[sym6(i, i)];
 #pragma unroll
     for (int j = 0; j < i; ++j) {
       double acc = 0.0;
 #pragma unroll
-      for (int k = j; k < i; ++k) acc += L[i][k] * Li[k][j];
+      for (int k = j; k < i; ++k) acc += L[sym6(i, k)] * Li[k][j];
       Li[i][j] = -acc * Li[i][i];
     }
   }
@@ -241,71 +271,83 @@ __device__ __forceinline__ void spd_inverse6(const double S[21], double Sinv[21]
     }
 }
 
-// One iteration of the per-point implicit update (elastic_viscoplastic.py:190-253).
-template <int NS>
+// One iteration of the per-point implicit update (elastic_viscoplastic.py:190-253),
+// accumulated one slip system at a time.  The per-system values live in LDS
+// columns (system s at col[s * kCBT]) rather than registers, so the loop over
+// systems need not be unrolled and the kernel keeps enough waves resident to
+// overlap its HBM traffic with the FP64 arithmetic.  On return S holds the
+// Cholesky factor of C^-1 + dt M, sig_new the updated stress, ep the plastic
+// strain, d the slip increment dt sum_s |rate_s|, and dgam (when non-null) the
+// slip increments rate_s dt.  The total and committed plastic strains are read
+// (components strided by ln) only after the loop over systems, which keeps
+// them out of the registers live during it.
+template <int NS, bool IP>
 __device__ __forceinline__ void evp_iterate(const EvpParams<NS>& q, const double sig[6],
-                                            const double g[NS], const double eps[6],
-                                            const double ep_old[6], const double g_old[NS],
-                                            double acc_old, double sig_new[6], double g_new[NS],
-                                            double ep[6], double dgam[NS], double Sinv[21],
-                                            double& diff) {
-  double rate[NS], w[NS];
+                                            const double* g, const double* eps_p,
+                                            const double* ep_old_p, long long ln,
+                                            double sig_new[6], double ep[6], double S[21],
+                                            double* dgam, double& diff, double& d) {
+  double epr[6];
 #pragma unroll
+  for (int m = 0; m < 6; ++m) epr[m] = 0.0;
+#pragma unroll
+  for (int k = 0; k < 21; ++k) S[k] = 0.0;
+  double dsum = 0.0;
+#pragma unroll 1
   for (int s = 0; s < NS; ++s) {
     double tau = 0.0;
 #pragma unroll
     for (int m = 0; m < 6; ++m) tau += q.P[s][m] * sig[m];
-    const double pref = q.gamma0 * powi_abs(tau / g[s], q.nint, q.n - 1.0) / g[s];
-    rate[s] = pref * tau;
-    w[s] = q.n * pref;
+    const double gs = g[s * kCBT];
+    const double pref = q.gamma0 * powi_abs<IP>(tau / gs, q.nint, q.n - 1.0) / gs;
+    const double rate = pref * tau;
+    const double w = q.n * pref;
+#pragma unroll
+    for (int m = 0; m < 6; ++m) epr[m] += rate * q.P[s][m];
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = a; b < 6; ++b) S[sym6(a, b)] += w * q.P[s][a] * q.P[s][b];
+    if (dgam != nullptr) dgam[s * kCBT] = rate * q.dt;
+    dsum += fabs(rate);
   }
 #pragma unroll
-  for (int m = 0; m < 6; ++m) {
-    double acc = 0.0;
+  for (int m = 0; m < 6; ++m) ep[m] = ep_old_p[m * ln] + epr[m] * q.dt;
 #pragma unroll
-    for (int s = 0; s < NS; ++s) acc += rate[s] * q.P[s][m];
-    ep[m] = ep_old[m] + acc * q.dt;
-  }
-  double S[21];
-#pragma unroll
-  for (int a = 0; a < 6; ++a)
-#pragma unroll
-    for (int b = a; b < 6; ++b) {
-      double acc = 0.0;
-#pragma unroll
-      for (int s = 0; s < NS; ++s) acc += w[s] * q.P[s][a] * q.P[s][b];
-      S[sym6(a, b)] = q.Cinv[sym6(a, b)] + q.dt * acc;
-    }
-  spd_inverse6(S, Sinv);
+  for (int k = 0; k < 21; ++k) S[k] = q.Cinv[k] + q.dt * S[k];
   double rc[6];
 #pragma unroll
   for (int a = 0; a < 6; ++a) {
     double acc = 0.0;
 #pragma unroll
     for (int b = 0; b < 6; ++b) acc += q.Cinv[sym6(a, b)] * sig[b];
-    rc[a] = acc - (eps[a] - ep[a]);
+    rc[a] = acc - (eps_p[a * ln] - ep[a]);
   }
+  spd_factor6(S);
+  double x[6];
+  spd_solve6(S, rc, x);
   double d2 = 0.0;
 #pragma unroll
   for (int a = 0; a < 6; ++a) {
-    double acc = 0.0;
-#pragma unroll
-    for (int b = 0; b < 6; ++b) acc += Sinv[sym6(a, b)] * rc[b];
-    sig_new[a] = sig[a] - acc;
-    d2 += acc * acc;
+    sig_new[a] = sig[a] - x[a];
+    d2 += x[a] * x[a];
   }
   diff = sqrt(d2);
-  // hardening (hardening_laws.py:18-90): one slip increment for all systems
-  double dsum = 0.0;
-#pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    dgam[s] = rate[s] * q.dt;
-    dsum += fabs(rate[s]);
-  }
-  const double d = q.dt * dsum;
+  d = q.dt * dsum;
+}
+
+// Slip resistances after a slip increment d from the committed state
+// (hardening_laws.py:18-90).
+// g_old and g_new are strided (system s at [s * stride]).
+template <int NS>
+__device__ __forceinline__ void evp_harden(const EvpParams<NS>& q, const double* g_old,
+                                           long long old_stride, double acc_old, double d,
+                                           double* g_new, long long new_stride) {
+#define G_OLD(s) g_old[(s) * old_stride]
+#define G_NEW(s) g_new[(s) * new_stride]
   if (q.hard == kLinear) {
 #pragma unroll
-    for (int s = 0; s < NS; ++s) g_new[s] = g_old[s] + q.hp[0] * d;
+    for (int s = 0; s < NS; ++s) G_NEW(s) = G_OLD(s) + q.hp[0] * d;
   } else if (q.hard == kVoce) {
     const double theta0 = q.hp[0], theta1 = q.hp[1], tau1 = q.hp[2];
     const double ratio = fabs(theta0 / tau1);
@@ -314,15 +356,20 @@ __device__ __forceinline__ void evp_iterate(const EvpParams<NS>& q, const double
     const double inc = theta1 * d - (ratio * tau1 - theta1) / ratio * e0 * (ed - 1.0) -
                        theta1 / ratio * e0 * (ed * (1.0 + ratio * (acc_old + d)) - 1.0 - ratio * acc_old);
 #pragma unroll
-    for (int s = 0; s < NS; ++s) g_new[s] = g_old[s] + inc;
+    for (int s = 0; s < NS; ++s) G_NEW(s) = G_OLD(s) + inc;
   } else if (q.hard == kArmstrongFrederick) {
     const double h = q.hp[0], r = q.hp[1];
 #pragma unroll
-    for (int s = 0; s < NS; ++s) g_new[s] = g_old[s] + (h - r * g_old[s]) * d / (1.0 + r * d);
+    for (int s = 0; s < NS; ++s) {
+      const double go = G_OLD(s);
+      G_NEW(s) = go + (h - r * go) * d / (1.0 + r * d);
+    }
   } else {
 #pragma unroll
-    for (int s = 0; s < NS; ++s) g_new[s] = g_old[s];
+    for (int s = 0; s < NS; ++s) G_NEW(s) = G_OLD(s);
   }
+#undef G_OLD
+#undef G_NEW
 }
 
 template <int NS>
@@ -335,7 +382,8 @@ struct EvpBuffers {
   double* eps_c;     // [6][n]
   double* gnorm;     // [n]
   double* sig[2];    // [6][n] ping-pong
-  double* g[2];      // [NS][n] ping-pong
+  double* dsl[2];    // [n] ping-pong slip increment d of the iteration
+  const double* g_trial;  // [NS][n] slip resistances of the first iteration
   unsigned char* conv;
   int* last;         // [n] last iteration each point ran
   double* ep_old;    // [6][n]
@@ -351,8 +399,7 @@ struct EvpBuffers {
 template <int NS>
 __global__ void __launch_bounds__(kCBT) k_evp_setup(int n, const int* idx, long long npts,
                                                      const double* rot9, const double* eps,
-                                                     const double* guess, const double* g_trial,
-                                                     EvpBuffers B) {
+                                                     const double* guess, EvpBuffers B) {
   const int i = blockIdx.x * kCBT + threadIdx.x;
   if (i >= n) return;
   const long long p = idx[i];
@@ -373,8 +420,6 @@ __global__ void __launch_bounds__(kCBT) k_evp_setup(int n, const int* idx, long 
   to_crystal(Rm, v, vc);
 #pragma unroll
   for (int c = 0; c < 6; ++c) B.sig[0][c * (long long)n + i] = vc[c];
-#pragma unroll
-  for (int s = 0; s < NS; ++s) B.g[0][s * (long long)n + i] = g_trial[s * (long long)n + i];
 }
 
 __global__ void k_evp_ctrl_reset(EvpControl* c) {
@@ -382,51 +427,51 @@ __global__ void k_evp_ctrl_reset(EvpControl* c) {
   for (int i = threadIdx.x; i < int(sizeof(EvpControl) / sizeof(int)); i += blockDim.x) w[i] = 0;
 }
 
+// State entering iteration it (1-based): the stress written by iteration it-1
+// (the initial guess for it = 1) and the slip resistances that iteration
+// produced (the trial values for it = 1), the latter into the LDS column g.
 template <int NS>
-__device__ __forceinline__ void load_point(const EvpBuffers& B, int n, int i, int buf, double sig[6],
-                                           double g[NS], double eps[6], double ep_old[6],
-                                           double g_old[NS], double& acc_old) {
+__device__ __forceinline__ void load_point(const EvpParams<NS>& q, const EvpBuffers& B, int n,
+                                           int i, int it, double sig[6], double* g) {
   const long long ln = n;
+  const int buf = (it - 1) & 1;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) {
-    sig[c] = B.sig[buf][c * ln + i];
-    eps[c] = B.eps_c[c * ln + i];
-    ep_old[c] = B.ep_old[c * ln + i];
-  }
+  for (int c = 0; c < 6; ++c) sig[c] = B.sig[buf][c * ln + i];
+  if (it == 1) {
 #pragma unroll
-  for (int s = 0; s < NS; ++s) {
-    g[s] = B.g[buf][s * ln + i];
-    g_old[s] = B.g_old[s * ln + i];
+    for (int s = 0; s < NS; ++s) g[s * kCBT] = B.g_trial[s * ln + i];
+  } else {
+    evp_harden<NS>(q, B.g_old + i, ln, B.acc_old[i], B.dsl[buf][i], g, kCBT);
   }
-  acc_old = B.acc_old[i];
 }
 
 // Iteration `it` (1-based) of every point that still iterates: all points
 // before the masked phase, only unconverged ones after it
 // (elastic_viscoplastic.py:116-158).  A point's state after iteration j lives
 // in ping-pong buffer j&1; last[i] records the last iteration it ran.
-template <int NS>
+template <int NS, bool IP>
 __global__ void __launch_bounds__(kCBT) k_evp_iter(int it, int n, const EvpParams<NS>* qg,
                                                     EvpBuffers B, EvpControl* ctrl) {
   if (ctrl->done) return;
   const int masked = ctrl->masked;
   __shared__ EvpParams<NS> q;
+  __shared__ double gsh[NS * kCBT];
   __shared__ int cnt;
   load_params(&q, qg);
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
   const int i = blockIdx.x * kCBT + threadIdx.x;
   if (i < n && !(masked && B.conv[i])) {
-    const int src = (it - 1) & 1, dst = it & 1;
-    double sig[6], g[NS], eps[6], ep_old[6], g_old[NS], acc_old;
-    load_point<NS>(B, n, i, src, sig, g, eps, ep_old, g_old, acc_old);
-    double sig_new[6], g_new[NS], ep[6], dgam[NS], Sinv[21], diff;
-    evp_iterate<NS>(q, sig, g, eps, ep_old, g_old, acc_old, sig_new, g_new, ep, dgam, Sinv, diff);
+    const int dst = it & 1;
     const long long ln = n;
+    double sig[6];
+    double* g = gsh + threadIdx.x;
+    load_point<NS>(q, B, n, i, it, sig, g);
+    double sig_new[6], ep[6], S[21], diff, d;
+    evp_iterate<NS, IP>(q, sig, g, B.eps_c + i, B.ep_old + i, ln, sig_new, ep, S, nullptr, diff, d);
 #pragma unroll
     for (int c = 0; c < 6; ++c) B.sig[dst][c * ln + i] = sig_new[c];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) B.g[dst][s * ln + i] = g_new[s];
+    B.dsl[dst][i] = d;
     const bool cv = evp_converged<NS>(q, diff, B.gnorm[i]);
     B.conv[i] = cv ? 1 : 0;
     B.last[i] = it;
@@ -460,7 +505,7 @@ __global__ void k_evp_control(int it, int n, EvpControl* ctrl) {
 
 // Outputs of each point's last iteration, recomputed from the state it started
 // from (bitwise the same stress and slip resistances as that iteration wrote).
-template <int NS, typename K>
+template <int NS, typename K, bool IP>
 __global__ void __launch_bounds__(kCBT) k_evp_outputs(int n, const int* idx, long long npts,
                                                        const double* rot9,
                                                        const EvpParams<NS>* qg, EvpBuffers B,
@@ -468,17 +513,23 @@ __global__ void __launch_bounds__(kCBT) k_evp_outputs(int n, const int* idx, lon
                                                        K* tangent) {
   if (ctrl->fail || !ctrl->done) return;
   __shared__ EvpParams<NS> q;
+  __shared__ double gsh[NS * kCBT];
+  __shared__ double dgsh[NS * kCBT];
   load_params(&q, qg);
   __syncthreads();
   const int i = blockIdx.x * kCBT + threadIdx.x;
   if (i >= n) return;
-  const int src = (B.last[i] - 1) & 1;
-  double sig[6], g[NS], eps[6], ep_old[6], g_old[NS], acc_old;
-  double sig_new[6], g_new[NS], ep[6], dgam[NS], Sinv[21], diff;
-  load_point<NS>(B, n, i, src, sig, g, eps, ep_old, g_old, acc_old);
-  evp_iterate<NS>(q, sig, g, eps, ep_old, g_old, acc_old, sig_new, g_new, ep, dgam, Sinv, diff);
-  const long long p = idx[i];
   const long long ln = n;
+  double sig[6];
+  double sig_new[6], ep[6], S[21], Sinv[21], diff, d;
+  double* g = gsh + threadIdx.x;
+  double* dgam = dgsh + threadIdx.x;
+  load_point<NS>(q, B, n, i, B.last[i], sig, g);
+  evp_iterate<NS, IP>(q, sig, g, B.eps_c + i, B.ep_old + i, ln, sig_new, ep, S, dgam, diff, d);
+  spd_inverse6(S, Sinv);
+  const double acc_old = B.acc_old[i];
+  evp_harden<NS>(q, B.g_old + i, ln, acc_old, d, B.g_pend + i, ln);
+  const long long p = idx[i];
   double Rm[36], vs[6], Ds[21];
   load_rotation(rot9, npts, p, Rm);
   to_specimen(Rm, sig_new, vs);
@@ -491,9 +542,9 @@ __global__ void __launch_bounds__(kCBT) k_evp_outputs(int n, const int* idx, lon
   for (int c = 0; c < 6; ++c) B.ep_pend[c * ln + i] = ep[c];
 #pragma unroll
   for (int s = 0; s < NS; ++s) {
-    B.g_pend[s * ln + i] = g_new[s];
-    B.gam_pend[s * ln + i] = B.gam_old[s * ln + i] + dgam[s];
-    asum += fabs(dgam[s]);
+    const double dg = dgam[s * kCBT];
+    B.gam_pend[s * ln + i] = B.gam_old[s * ln + i] + dg;
+    asum += fabs(dg);
   }
   B.acc_pend[i] = acc_old + asum;
 }
@@ -632,7 +683,7 @@ class EvpModel final : public Model<K> {
     alloc(B_.gnorm, n);
     for (int k = 0; k < 2; ++k) {
       alloc(B_.sig[k], 6 * n);
-      alloc(B_.g[k], NS * n);
+      alloc(B_.dsl[k], n);
     }
     MTL_HIP(hipMalloc(&B_.conv, n));
     MTL_HIP(hipMalloc(&B_.last, n * sizeof(int)));
@@ -654,7 +705,7 @@ class EvpModel final : public Model<K> {
     MTL_HIP(hipHostMalloc(&hctrl_, sizeof(EvpControl), hipHostMallocDefault));
   }
   ~EvpModel() override {
-    for (double* p : {B_.eps_c, B_.gnorm, B_.sig[0], B_.sig[1], B_.g[0], B_.g[1], B_.ep_old,
+    for (double* p : {B_.eps_c, B_.gnorm, B_.sig[0], B_.sig[1], B_.dsl[0], B_.dsl[1], B_.ep_old,
                       B_.g_old, B_.gam_old, B_.acc_old, B_.ep_pend, B_.g_pend, B_.gam_pend,
                       B_.acc_pend, ep_tr_, g_tr_, gam_tr_, acc_tr_})
       if (p) (void)hipFree(p);
@@ -692,8 +743,9 @@ class EvpModel final : public Model<K> {
     *qh_ = q_;
     MTL_HIP(hipMemcpyAsync(qd_, qh_, sizeof(EvpParams<NS>), hipMemcpyHostToDevice, s));
     const int blocks = ceil_div(n, kCBT);
+    B_.g_trial = g_tr_;
     hipLaunchKernelGGL(k_evp_setup<NS>, dim3(blocks), dim3(kCBT), 0, s, n, this->idx_, this->npts_,
-                       rot9, eps, guess, g_tr_, B_);
+                       rot9, eps, guess, B_);
     MTL_LAUNCH_CHECK();
     hipLaunchKernelGGL(k_evp_ctrl_reset, dim3(1), dim3(64), 0, s, ctrl_);
     MTL_LAUNCH_CHECK();
@@ -705,7 +757,12 @@ class EvpModel final : public Model<K> {
       for (int k = 0; k < chunk && it < kEvpMaxIterations; ++k) {
         ++it;
         profiler().begin(s);
-        hipLaunchKernelGGL(k_evp_iter<NS>, dim3(blocks), dim3(kCBT), 0, s, it, n, qd_, B_, ctrl_);
+        if (q_.nint >= 0)
+          hipLaunchKernelGGL((k_evp_iter<NS, true>), dim3(blocks), dim3(kCBT), 0, s, it, n, qd_, B_,
+                             ctrl_);
+        else
+          hipLaunchKernelGGL((k_evp_iter<NS, false>), dim3(blocks), dim3(kCBT), 0, s, it, n, qd_, B_,
+                             ctrl_);
         MTL_LAUNCH_CHECK();
         profiler().end(kKEvpIter, s);
         hipLaunchKernelGGL(k_evp_control, dim3(1), dim3(64), 0, s, it, n, ctrl_);
@@ -725,8 +782,12 @@ class EvpModel final : public Model<K> {
     }
     if (hctrl_->fail || !hctrl_->done) return 0;
     profiler().begin(s);
-    hipLaunchKernelGGL((k_evp_outputs<NS, K>), dim3(blocks), dim3(kCBT), 0, s, n, this->idx_,
-                       this->npts_, rot9, qd_, B_, ctrl_, sig, tangent);
+    if (q_.nint >= 0)
+      hipLaunchKernelGGL((k_evp_outputs<NS, K, true>), dim3(blocks), dim3(kCBT), 0, s, n,
+                         this->idx_, this->npts_, rot9, qd_, B_, ctrl_, sig, tangent);
+    else
+      hipLaunchKernelGGL((k_evp_outputs<NS, K, false>), dim3(blocks), dim3(kCBT), 0, s, n,
+                         this->idx_, this->npts_, rot9, qd_, B_, ctrl_, sig, tangent);
     MTL_LAUNCH_CHECK();
     profiler().end(kKEvpFinish, s);
     // adopt the call's outputs as the trial state (elastic_viscoplastic.py:175-182)
