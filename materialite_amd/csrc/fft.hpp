@@ -27,7 +27,19 @@ struct AxisPlan {
   int npass;
   int radix[kMaxPasses];
   int ns[kMaxPasses];
+  // ceil(2^32 / ns): j / ns = (j * ns_magic) >> 32 exactly for j, ns < 2^16
+  unsigned long long ns_magic[kMaxPasses];
 };
+
+// q = n / d for 0 <= n < 2^16 and 1 <= d <= 2^16, with magic = ceil(2^32 / d):
+// n * magic / 2^32 exceeds n / d by less than n / 2^32 < 1 / d, which cannot
+// carry it past the next integer.
+__host__ __device__ inline unsigned long long div_magic(unsigned d) {
+  return ((1ULL << 32) + d - 1) / d;
+}
+__device__ __forceinline__ int fast_div(int n, unsigned long long magic) {
+  return int((unsigned long long)(unsigned)n * magic >> 32);
+}
 
 // --------------------------------------------------------------- small DFTs
 template <typename T, int R>
@@ -180,23 +192,26 @@ __device__ __forceinline__ void butterfly(const cplx<T>* __restrict__ src, cplx<
 // Transform M interleaved lines held in `a` (row stride S >= M complex values),
 // using `b` as the second buffer.  Returns the buffer that holds the result.
 // Ends with a barrier.  The radix switch sits inside the butterfly loop so that
-// the per-pass index arithmetic is shared by all radices.
+// the per-pass index arithmetic is shared by all radices.  Butterfly
+// bf = j * M + m of a thread advances by BT per step; (j, m) is stepped
+// incrementally instead of divided out.
 template <typename T, int BT>
 __device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int S,
                                               const AxisPlan* __restrict__ pl, const cplx<T>* tw,
                                               bool inv) {
   const int npass = pl->npass;
   const int L = pl->L;
+  const int dj = BT / M, dm = BT - dj * M;
+  const int j0 = int(threadIdx.x) / M, m0 = int(threadIdx.x) - j0 * M;
   for (int p = 0; p < npass; ++p) {
     const int R = pl->radix[p];
     const int Ns = pl->ns[p];
+    const unsigned long long mg = pl->ns_magic[p];
     const int per_line = L / R;
-    const int total = M * per_line;
     const int tstride = per_line / Ns;
-    for (int bf = threadIdx.x; bf < total; bf += BT) {
-      const int j = bf / M;
-      const int m = bf - j * M;
-      const int jq = j / Ns;
+    int m = m0;
+    for (int j = j0; j < per_line;) {
+      const int jq = fast_div(j, mg);
       const int kk = j - jq * Ns;
       switch (R) {
         case 8: butterfly<T, 8>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
@@ -206,6 +221,12 @@ __device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int
         case 5: butterfly<T, 5>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
         case 7: butterfly<T, 7>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
         default: break;  // rejected on the host when the plan is built
+      }
+      j += dj;
+      m += dm;
+      if (m >= M) {
+        m -= M;
+        ++j;
       }
     }
     __syncthreads();
@@ -228,6 +249,7 @@ inline AxisPlan make_axis_plan(int L) {
       if (p.npass >= kMaxPasses) throw Error("FFT plan: too many passes");
       p.radix[p.npass] = r;
       p.ns[p.npass] = ns;
+      p.ns_magic[p.npass] = div_magic(unsigned(ns));
       ns *= r;
       rem /= r;
       ++p.npass;
@@ -236,6 +258,7 @@ inline AxisPlan make_axis_plan(int L) {
   if (rem != 1)
     throw Error("grid dimension " + std::to_string(L) +
                 " has a prime factor above 7; the FFT supports radices 2, 3, 4, 5, 7, 8");
+  if (L >= (1 << 16)) throw Error("grid dimension " + std::to_string(L) + " too large");
   return p;
 }
 
