@@ -62,6 +62,16 @@ __device__ __forceinline__ int xcd_logical_block(int b, int nblocks) {
   return (b % nx) * (nblocks / nx) + b / nx;
 }
 
+// Workgroup barrier that orders LDS accesses only.  __syncthreads() also waits
+// for every outstanding global load of the wave; the pass kernels keep the
+// next tile's loads in flight across their FFT barriers, so they synchronise
+// with this one (fences restricted to the local address space).
+__device__ __forceinline__ void lds_barrier() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // block-wide sum of a double, result valid in thread 0
 template <int BT>
 __device__ __forceinline__ double block_sum(double v, double* red) {

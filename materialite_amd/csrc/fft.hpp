@@ -229,7 +229,7 @@ __device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int
         ++j;
       }
     }
-    __syncthreads();
+    lds_barrier();
     cplx<T>* t = a;
     a = b;
     b = t;
