@@ -47,9 +47,66 @@ def parse():
     ap.add_argument("--compare-fp64", type=int, default=1,
                     help="also time the plain FP64 form on the same increments")
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--stream-requests", type=int, default=32,
+                    help="requests of the mixed-size serving stream (0 = skip)")
+    ap.add_argument("--stream-rate", type=float, default=12.0,
+                    help="Poisson arrival rate of the serving stream (requests/s)")
+    ap.add_argument("--stream-slots", type=int, default=8,
+                    help="solves in flight at once in the serving stream")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-iterations", type=int, default=4, help=argparse.SUPPRESS)
     return ap.parse_args()
+
+
+def serving_stream(nreq, rate, slots):
+    """Mixed-size request stream: Poisson arrivals at `rate`, request j drawn
+    from seed 20264000 + j (grid 32/48/64/96^3, 50-500 grains, elastic or EVP,
+    one of six load directions, 20 steps of 0.5 s), continuous batching with
+    admission by HBM footprint.  Materials are generated before the clock
+    starts; latency runs from arrival to the returned Material."""
+    import numpy as np
+    import torch
+
+    from materialite_amd import serving
+
+    specs = [serving.draw_stream_spec(20264000 + j) for j in range(nreq)]
+    print(f"[bench] stream: building {nreq} requests", file=sys.stderr, flush=True)
+    reqs = [serving.build_stream_request(sp) for sp in specs]
+    gaps = np.random.default_rng(20264999).exponential(1.0 / rate, nreq)
+    budget = int(0.5 * torch.cuda.mem_get_info()[0])
+    server = serving.RveServer(hbm_budget=budget, max_slots=slots)
+    t0 = time.perf_counter()
+    arrivals = t0 + np.cumsum(gaps) - gaps[0]
+    futs = []
+    for r, ta in zip(reqs, arrivals):
+        delay = ta - time.perf_counter()
+        if delay > 0:
+            time.sleep(delay)
+        r.arrival = ta
+        futs.append(server.submit(r))
+    done = 0
+    for f in futs:
+        f.result()
+        done += 1
+        if done % 4 == 0:
+            print(f"[bench] stream: {done}/{nreq} solves done", file=sys.stderr, flush=True)
+    server.close()
+    summ = serving.latency_summary(reqs)
+    vi = sum(sp.n ** 3 * r.result["krylov"] for sp, r in zip(specs, reqs))
+    span = max(r.finished for r in reqs) - min(r.arrival for r in reqs)
+    summ.update({
+        "arrival_rate_per_s": rate,
+        "slots": slots,
+        "hbm_budget_gb": budget / 1e9,
+        "peak_admitted_gb": server.peak_bytes / 1e9,
+        "peak_in_flight": server.peak_in_flight,
+        "voxel_iters_per_s": vi / span,
+        "mix": {"evp": sum(sp.model == "evp" for sp in specs),
+                "grids": {str(n): sum(sp.n == n for sp in specs) for n in (32, 48, 64, 96)}},
+        "precision": "mixed",
+        "seeds": "request j: 20264000 + j; arrivals: 20264999",
+    })
+    return summ
 
 
 def workload(n, grains):
@@ -227,6 +284,8 @@ def main():
             "seconds": rec["seconds"],
             "ms_per_step": 1e3 * rec["seconds"] / max(rec["steps"], 1),
         }
+    stream = (serving_stream(args.stream_requests, args.stream_rate, args.stream_slots)
+              if args.stream_requests > 0 else None)
     cpu = cpu_baseline(n, args.grains, args.cpu_iterations) if args.cpu_baseline else None
     line = {
         "metric": "voxel-iters/s at 128³ EVP, FP32-refined vs FP64; p50/p99 solve latency mixed",
@@ -262,6 +321,7 @@ def main():
         "kernels_ms": {k: round(v[0], 3) for k, v in prof.items()},
         "kernels_launches": {k: int(v[1]) for k, v in prof.items()},
         "fp64_form": fp64,
+        "serving": stream,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -1,0 +1,279 @@
+"""Serving stream: concurrent RVE homogenization solves on one GPU.
+
+A ``RveServer`` takes independent ``SmallStrainFFT`` solves as they arrive and
+runs them with continuous batching:
+
+* every admitted request owns a solver handle -- its own HIP stream, and its
+  fields, tangents, Krylov vectors and hardening state resident in HBM for all
+  of its load steps;
+* a request is admitted (FIFO) as soon as a slot is free and its HBM footprint
+  fits the budget left by the requests in flight; requests that arrive while
+  others run join them without waiting for the batch to drain;
+* each admitted request runs in a worker thread; the library's calls release
+  the GIL (ctypes), so the device work of all requests in flight overlaps on
+  the GPU, each on its own stream.
+
+A solve's arithmetic does not depend on what else is in flight (no shared
+buffers, no data-dependent scheduling across handles), so a result is the same
+alone and in a mix.  Per-request times (arrival, admission, completion) give
+the latency distribution reported beside throughput.
+
+The reference has no serving layer; this module is the request loop that the
+mixed-size stream workload runs on.
+"""
+
+import threading
+import time
+from collections import deque
+from concurrent.futures import Future
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# HBM bytes per voxel of one solver handle (allocations in csrc/solver.hip,
+# spectral.hip, minres.hip, constitutive.hip), h = nzc / nz ~ 1/2:
+#   FP64 strain/stress fields and work vectors: 7 x 6 doubles + rotation 9 doubles
+#   FP64 spectrum (residual projection): 6 complex doubles x h
+#   Krylov FP32/FP64 spectrum (mixed only): 6 complex floats x h
+#   MINRES: 6 Mandel vectors of the Krylov type
+#   tangent and committed tangent: 2 x 21 packed of the Krylov type
+_FIELDS = 7 * 6 * 8 + 9 * 8
+# per point of a model: elastic index; EVP iteration, committed, pending and
+# trial state (EvpModel in csrc/constitutive.hip): per slip system g and
+# gamma for committed, pending and trial states; per point 42 doubles (crystal
+# strain, |guess|, 2 stresses, 2 slip increments, 3 x (plastic strain,
+# accumulated slip)), a flag, the last iteration and the point index
+_ELASTIC_POINT = 4
+_EVP_POINT_PER_SYSTEM = 6 * 8
+_EVP_POINT_FIXED = 42 * 8 + 1 + 4 + 4
+
+
+def footprint_bytes(dims, precision="mixed", evp_points=0, num_slip_systems=12):
+    """Upper estimate of the device bytes one solve holds (checked against the
+    device's free-memory change in the GPU tests)."""
+    n = int(np.prod(dims))
+    nz = int(dims[2])
+    h = ((nz + 1) // 2) / nz
+    k = 4 if precision == "mixed" else 8
+    per_voxel = _FIELDS + 6 * 16 * h + (6 * 8 * h if precision == "mixed" else 0)
+    per_voxel += 6 * 6 * k + 2 * 21 * k + _ELASTIC_POINT
+    evp = evp_points * (_EVP_POINT_FIXED + num_slip_systems * _EVP_POINT_PER_SYSTEM)
+    # allocation granularity (about 30 device allocations per handle) and
+    # per-handle fixed buffers
+    return int(1.05 * (n * per_voxel + evp)) + (32 << 20)
+
+
+@dataclass
+class Request:
+    """One RVE solve: ``run()`` performs it (e.g. a bound ``SmallStrainFFT.run``)."""
+    run: callable
+    footprint: int
+    tag: object = None
+    arrival: float = None
+    admitted: float = None
+    finished: float = None
+    result: object = None
+    error: BaseException = None
+    future: Future = field(default_factory=Future)
+
+    @property
+    def latency(self):
+        return None if self.finished is None else self.finished - self.arrival
+
+
+class RveServer:
+    """Continuous-batching request loop with admission by HBM footprint."""
+
+    def __init__(self, hbm_budget, max_slots=8):
+        if max_slots < 1:
+            raise ValueError("max_slots must be >= 1")
+        self.hbm_budget = int(hbm_budget)
+        self.max_slots = int(max_slots)
+        self._queue = deque()
+        self._in_flight = 0
+        self._bytes = 0
+        self.peak_bytes = 0
+        self.peak_in_flight = 0
+        self._cv = threading.Condition()
+        self._closed = False
+        self._threads = []
+        self.completed = []
+        self._dispatcher = threading.Thread(target=self._dispatch, daemon=True)
+        self._dispatcher.start()
+
+    # --------------------------------------------------------------- client
+    def submit(self, request):
+        if request.footprint > self.hbm_budget:
+            raise ValueError(f"request needs {request.footprint} bytes, budget is {self.hbm_budget}")
+        with self._cv:
+            if self._closed:
+                raise RuntimeError("server is closed")
+            request.arrival = time.perf_counter() if request.arrival is None else request.arrival
+            self._queue.append(request)
+            self._cv.notify_all()
+        return request.future
+
+    def close(self, wait=True):
+        with self._cv:
+            self._closed = True
+            self._cv.notify_all()
+        if wait:
+            self._dispatcher.join()
+            for t in list(self._threads):
+                t.join()
+
+    # --------------------------------------------------------------- loop
+    def _fits(self, req):
+        return (self._in_flight < self.max_slots
+                and self._bytes + req.footprint <= self.hbm_budget)
+
+    def _dispatch(self):
+        while True:
+            with self._cv:
+                while not (self._queue and self._fits(self._queue[0])):
+                    if self._closed and not self._queue:
+                        return
+                    self._cv.wait()
+                req = self._queue.popleft()
+                self._in_flight += 1
+                self._bytes += req.footprint
+                self.peak_bytes = max(self.peak_bytes, self._bytes)
+                self.peak_in_flight = max(self.peak_in_flight, self._in_flight)
+            req.admitted = time.perf_counter()
+            t = threading.Thread(target=self._work, args=(req,), daemon=True)
+            self._threads.append(t)
+            t.start()
+
+    def _work(self, req):
+        try:
+            req.result = req.run()
+        except BaseException as e:  # reported through the future
+            req.error = e
+        req.finished = time.perf_counter()
+        with self._cv:
+            self._in_flight -= 1
+            self._bytes -= req.footprint
+            self.completed.append(req)
+            self._cv.notify_all()
+        if req.error is not None:
+            req.future.set_exception(req.error)
+        else:
+            req.future.set_result(req.result)
+
+
+def latency_summary(requests):
+    """p50/p99 latency (s), mean queueing delay and solves/s over finished requests."""
+    done = [r for r in requests if r.finished is not None and r.error is None]
+    if not done:
+        return {"count": 0}
+    lat = np.array([r.latency for r in done])
+    wait = np.array([r.admitted - r.arrival for r in done])
+    span = max(r.finished for r in done) - min(r.arrival for r in done)
+    return {
+        "count": len(done),
+        "p50_s": float(np.percentile(lat, 50)),
+        "p99_s": float(np.percentile(lat, 99)),
+        "mean_queue_s": float(wait.mean()),
+        "solves_per_s": len(done) / span if span > 0 else float("inf"),
+    }
+
+
+# ------------------------------------------------------------------ workload
+_SHEAR_SLOT = {"yz": 3, "xz": 4, "xy": 5}
+DIRECTIONS = ("xx", "yy", "zz", "xy", "yz", "xz")
+
+
+def stream_load_schedule(direction, rate=1.0e-3):
+    """Uniaxial strain rate along a normal direction (stress-free laterally),
+    or a single shear Mandel component sqrt(2) * rate with the other five
+    stress components held at zero."""
+    from .models.small_strain_fft import LoadSchedule
+    from .tensor import Order2SymmetricTensor
+
+    if direction in ("xx", "yy", "zz"):
+        return LoadSchedule.from_constant_uniaxial_strain_rate(rate, direction[0])
+    slot = _SHEAR_SLOT[direction]
+    strain_rate = np.zeros(6)
+    strain_rate[slot] = np.sqrt(2.0) * rate
+    mask = np.ones(6)
+    mask[slot] = 0.0
+    return LoadSchedule.from_constant_rates(Order2SymmetricTensor(strain_rate),
+                                            Order2SymmetricTensor.zero(), mask)
+
+
+def config3_evp():
+    """FCC crystal plasticity of the 128^3 workload: cubic C11, C12, C44 =
+    250, 150, 120 GPa (in MPa), 12 octahedral systems, reference rate 1e-3/s,
+    n = 20, g0 = 50 MPa, Voce tau_1 = 100, theta_0 = 500, theta_1 = 0."""
+    from .models.small_strain_fft import ElasticViscoplastic, voce
+    from .slip_system import SlipSystem
+    from .tensor import Order4SymmetricTensor
+
+    c = Order4SymmetricTensor.from_cubic_constants(250e3, 150e3, 120e3)
+    return ElasticViscoplastic(c, SlipSystem.octahedral(), reference_slip_rate=1e-3,
+                               rate_exponent=20.0, slip_resistance=50.0, hardening_function=voce,
+                               hardening_properties={"tau_1": 100.0, "theta_0": 500.0,
+                                                     "theta_1": 0.0})
+
+
+def cubic_elastic():
+    from .models.small_strain_fft import Elastic
+    from .tensor import Order4SymmetricTensor
+
+    return Elastic(Order4SymmetricTensor.from_cubic_constants(250e3, 150e3, 120e3))
+
+
+@dataclass
+class StreamSpec:
+    """One request of the mixed-size stream, drawn from its own seed."""
+    seed: int
+    n: int
+    grains: int
+    model: str       # "elastic" or "evp"
+    direction: str
+    steps: int = 20
+    dt: float = 0.5
+
+
+def draw_stream_spec(seed, sizes=(32, 48, 64, 96), grains=(50, 500), steps=20, dt=0.5):
+    rng = np.random.default_rng(seed)
+    n = int(rng.choice(sizes))
+    g = int(rng.integers(grains[0], grains[1] + 1))
+    model = "evp" if rng.random() < 0.5 else "elastic"
+    direction = DIRECTIONS[int(rng.integers(len(DIRECTIONS)))]
+    return StreamSpec(seed=seed, n=n, grains=g, model=model, direction=direction, steps=steps, dt=dt)
+
+
+def build_stream_request(spec, precision="mixed", keep_material=False):
+    """Material (periodic Voronoi, Haar-random orientations from the spec's
+    seed), model and schedule of one request; returns a Request whose run()
+    performs the solve and returns the volume-averaged stress (Mandel, FP64)
+    and the Krylov iteration count, plus the stress field when asked."""
+    from .material import Material
+    from .microstructure import grain_rotations, periodic_voronoi
+    from .models.small_strain_fft import SmallStrainFFT
+    from .tensor import Orientation
+
+    rng = np.random.default_rng(spec.seed + 1)
+    grain_of_point, _ = periodic_voronoi([spec.n] * 3, spec.grains, rng)
+    rot = grain_rotations(spec.grains, rng)
+    material = Material(dimensions=[spec.n] * 3).create_fields({"grain": grain_of_point})
+    material = material.create_regional_fields(
+        "grain", {"grain": np.arange(spec.grains), "orientation": Orientation(rot, "p")})
+    model = config3_evp() if spec.model == "evp" else cubic_elastic()
+    solver = SmallStrainFFT(stream_load_schedule(spec.direction), end_time=spec.steps * spec.dt,
+                            initial_time_increment=spec.dt, max_time_increment=spec.dt,
+                            constitutive_model=model, precision=precision)
+
+    def run():
+        out = solver.run(material)
+        stress = out.extract("stress").components
+        res = {"mean_stress": stress.mean(axis=0), "krylov": solver.stats["krylov_iterations"],
+               "steps": solver.stats["steps"]}
+        if keep_material:
+            res["stress"] = stress
+        return res
+
+    evp_points = spec.n ** 3 if spec.model == "evp" else 0
+    return Request(run=run, footprint=footprint_bytes([spec.n] * 3, precision, evp_points),
+                   tag=spec)
