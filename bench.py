@@ -53,6 +53,13 @@ def parse():
                     help="Poisson arrival rate of the serving stream (requests/s)")
     ap.add_argument("--stream-slots", type=int, default=8,
                     help="solves in flight at once in the serving stream")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="MiB copied by a plain device copy before the workload (the reference "
+                         "dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
+                    help="print the per-dispatch HBM traffic summary of two rocprofv3 --pmc runs")
+    ap.add_argument("--summarize-trace", metavar="KERNEL_TRACE_CSV",
+                    help="print the per-kernel table of a rocprofv3 --kernel-trace run")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-iterations", type=int, default=4, help=argparse.SUPPRESS)
     return ap.parse_args()
@@ -181,6 +188,92 @@ def time_increments(material, precision, warmup, steps, profile):
     return rec
 
 
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_evp128_mixed.txt")
+KERNEL_SYMBOLS = {"z_forward": "k_z_forward<float, 1, float", "y_pass": "k_y_pass<float",
+                  "x_ghat": "k_x_ghat<float", "z_inverse": "k_z_inverse<float, 1, float",
+                  "evp_iteration": "k_evp_iter<12, true"}
+
+
+def summarize_pmc(fetch_csv, write_csv):
+    """Per-dispatch FETCH_SIZE and WRITE_SIZE (KiB) of the pass kernels and of
+    the copy probe, scaled so that the copy reads and writes exactly its size."""
+    import csv
+    import re
+    from collections import defaultdict
+
+    def load(path, counter):
+        per = defaultdict(list)
+        for r in csv.DictReader(open(path)):
+            if r["Counter_Name"] != counter:
+                continue
+            per[r["Kernel_Name"]].append(float(r["Counter_Value"]) * 1024.0)
+        return per
+
+    fetch, write = load(fetch_csv, "FETCH_SIZE"), load(write_csv, "WRITE_SIZE")
+    copy = [k for k in fetch if "elementwise" in k]
+    big = max(copy, key=lambda k: max(fetch[k]))
+    probe = max(fetch[big])  # the largest copy dispatch is the probe
+    probe_w = max(write[big])
+    return big, probe, probe_w, fetch, write, re
+
+
+def print_pmc_summary(fetch_csv, write_csv, probe_mib):
+    big, probe_f, probe_w, fetch, write, re = summarize_pmc(fetch_csv, write_csv)
+    size = probe_mib * 2 ** 20
+    sf, sw = size / probe_f, size / probe_w
+    print(f"copy probe `{re.sub(r'[(<].*', '', big)}`: {probe_mib} MiB read and written; "
+          f"FETCH_SIZE read {probe_f / 2**20:.1f} MiB, WRITE_SIZE {probe_w / 2**20:.1f} MiB; "
+          f"scale factors fetch {sf:.4f}, write {sw:.4f}\n")
+    print("| kernel | dispatches | FETCH bytes/dispatch (scaled) | WRITE bytes/dispatch (scaled) |")
+    print("|---|---:|---:|---:|")
+    for name, sym in KERNEL_SYMBOLS.items():
+        ks = [k for k in fetch if sym in k]
+        if not ks:
+            continue
+        f = [v for k in ks for v in fetch[k]]
+        w = [v for k in ks for v in write.get(k, [])]
+        # dispatches past convergence return at once: keep those that moved data
+        thr = 0.05 * max(f)
+        f = [v for v in f if v > thr]
+        w = [v for v in w if v > 0.05 * max(w)] if w else [0.0]
+        print(f"| {name} | {len(f)} | {sf * sum(f) / len(f):.0f} | {sw * sum(w) / len(w):.0f} |")
+
+
+def print_trace_summary(trace_csv):
+    """Per-kernel totals of a kernel trace.  'working' averages leave out the
+    dispatches enqueued past convergence, which return within a few
+    microseconds (shorter than 20 % of the kernel's longest dispatch)."""
+    import csv
+    import re
+    from collections import defaultdict
+
+    durs = defaultdict(list)
+    for r in csv.DictReader(open(trace_csv)):
+        name = r["Kernel_Name"].replace("mtl::(anonymous namespace)::", "").replace("mtl::", "")
+        name = re.sub(r"\(.*", "", name.replace("void ", ""))
+        durs[name].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+    total = sum(sum(v) for v in durs.values())
+    print("| kernel | dispatches | total us | avg us | working dispatches | working avg us | % |")
+    print("|---|---:|---:|---:|---:|---:|---:|")
+    for name, v in sorted(durs.items(), key=lambda kv: -sum(kv[1]))[:28]:
+        w = [d for d in v if d > 0.2 * max(v)]
+        print(f"| `{name}` | {len(v)} | {sum(v):.0f} | {sum(v) / len(v):.1f} | {len(w)} | "
+              f"{sum(w) / len(w):.1f} | {100 * sum(v) / total:.1f} |")
+
+
+def measured_traffic(kernel):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
+    (scaled FETCH + WRITE), or None."""
+    try:
+        for line in open(TRAFFIC_FILE):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == 4 and cells[0] == kernel:
+                return float(cells[2]) + float(cells[3])
+    except (OSError, ValueError):
+        return None
+    return None
+
+
 def kernel_bytes(n, nzc_frac, precision):
     """Algorithmic HBM bytes per launch of each pass kernel (FFT/Krylov type)."""
     esz = 4 if precision == "mixed" else 8
@@ -248,6 +341,12 @@ def main():
     if args.cpu_baseline_child:
         cpu_baseline_child(args.grid, args.grains, args.cpu_iterations)
         return
+    if args.summarize_trace:
+        print_trace_summary(args.summarize_trace)
+        return
+    if args.summarize_pmc:
+        print_pmc_summary(args.summarize_pmc[0], args.summarize_pmc[1], args.copy_probe or 512)
+        return
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}): this build has no "
               "multi-GPU path", file=sys.stderr)
@@ -256,6 +355,13 @@ def main():
     import torch
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.copy_probe:
+        src = torch.ones(args.copy_probe * 2 ** 18, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            torch.mul(src, 1.0, out=dst)  # one elementwise kernel: reads and writes the size
+        torch.cuda.synchronize()
+        del src, dst
     n = args.grid
     npts = n ** 3
     material = build_material(n, args.grains)
@@ -315,7 +421,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": measured_traffic(best) if args.precision == "mixed" and n == 128 else None,
             "mean_launch_us": mean_s * 1e6,
         },
         "kernels_ms": {k: round(v[0], 3) for k, v in prof.items()},
