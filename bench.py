@@ -53,6 +53,10 @@ def parse():
                     help="Poisson arrival rate of the serving stream (requests/s)")
     ap.add_argument("--stream-slots", type=int, default=8,
                     help="solves in flight at once in the serving stream")
+    ap.add_argument("--precision-study", type=int, default=1,
+                    help="64^3 elastic, uniaxial x then xy shear: mixed vs FP64 stress error")
+    ap.add_argument("--large", type=int, default=1,
+                    help="also run the 256^3 two-phase EVP solve (mixed and FP64; about a minute)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB copied by a plain device copy before the workload (the reference "
                          "dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -116,23 +120,132 @@ def serving_stream(nreq, rate, slots):
     return summ
 
 
-def workload(n, grains):
+def bytes_per_vi(nz, precision):
+    """Algorithmic HBM bytes per voxel-iteration of the minimal fused pipeline:
+    tangent (21), Krylov input and output (6 + 6), four passes over the
+    6-component half spectrum, and eight Mandel vectors of MINRES recurrences."""
+    k = 4 if precision == "mixed" else 8
+    h = (nz // 2 + 1) / nz
+    return 21 * k + 12 * k + 4 * 6 * 2 * k * h + 8 * 6 * k
+
+
+def _timed_run(material, schedule, model, precision, end_time, dt, **run_kw):
+    """Run a whole solve; time increments 2..end (the first one also builds the
+    solver state) and return (stress field, Krylov iterations, seconds)."""
+    from materialite_amd.models.small_strain_fft import SmallStrainFFT
+
+    rec = {"t0": None, "krylov": 0}
+
+    def cb(info):
+        if info["step"] == 1:
+            rec["t0"] = time.perf_counter()
+        else:
+            rec["krylov"] += info["krylov_iterations"]
+        rec["t1"] = time.perf_counter()
+        return False
+
+    solver = SmallStrainFFT(schedule, end_time=end_time, initial_time_increment=dt,
+                            max_time_increment=dt, constitutive_model=model, precision=precision)
+    out = solver.run(material, increment_callback=cb, **run_kw)
+    return out.extract("stress").components, rec["krylov"], rec["t1"] - rec["t0"]
+
+
+def precision_study(n=64, grains=200, steps=10):
+    """64^3 elastic, 200 periodic-Voronoi grains (seed 20260002): uniaxial x,
+    then pure xy shear, 10 steps of dt = 1 each, tolerances 1e-8 (Newton) and
+    1e-10 (MINRES); mixed vs plain FP64 form.  Error = max |sigma_mixed -
+    sigma_fp64| / max |sigma_fp64| over the whole field."""
+    import numpy as np
+
+    from materialite_amd import serving
+    from materialite_amd.models.small_strain_fft import LoadSchedule
+
+    material = build_material(n, grains, seed=20260002)
+    loads = {"uniaxial_x": LoadSchedule.from_constant_uniaxial_strain_rate(1e-3, "x"),
+             "shear_xy": serving.stream_load_schedule("xy")}
+    out = {"workload": f"{n}^3 elastic cubic, {grains} grains, {steps} steps of dt 1 per load, "
+                       "tolerances 1e-8 / 1e-10"}
+    for name, ls in loads.items():
+        res = {}
+        for prec in ("mixed", "fp64"):
+            print(f"[bench] precision study: {name} {prec}", file=sys.stderr, flush=True)
+            res[prec] = _timed_run(material, ls, serving.cubic_elastic(), prec, float(steps), 1.0,
+                                   global_tolerance=1e-8, linear_solver_tolerance=1e-10)
+        (sm, km, tm), (sd, kd, td) = res["mixed"], res["fp64"]
+        vim, vid = n ** 3 * km / tm, n ** 3 * kd / td
+        out[name] = {
+            "max_rel_stress_error": float(np.abs(sm - sd).max() / np.abs(sd).max()),
+            "krylov_mixed": int(km), "krylov_fp64": int(kd),
+            "vi_per_s_mixed": vim, "vi_per_s_fp64": vid,
+            "gb_per_s_mixed": vim * bytes_per_vi(n, "mixed") / 1e9,
+            "gb_per_s_fp64": vid * bytes_per_vi(n, "fp64") / 1e9,
+        }
+    return out
+
+
+def large_two_phase(n=256, grains=1000, steps=30, dt=0.2):
+    """256^3 two-phase EVP: 1000-grain FCC matrix (seed 20260005) with 10 %
+    volume fraction of periodic non-overlapping spheres of radius 8, elastic
+    isotropic with K, mu = 10 x the matrix Voigt averages; uniaxial z 1e-3/s,
+    30 steps of 0.2 s; mixed vs plain FP64 form."""
+    import numpy as np
+    import pandas as pd
+
+    from materialite_amd import Material, Order4SymmetricTensor, Orientation, serving
+    from materialite_amd.microstructure import grain_rotations, periodic_spheres, periodic_voronoi
+    from materialite_amd.models.small_strain_fft import Elastic, LoadSchedule
+
+    rng = np.random.default_rng(20260005)
+    grain_of_point, _ = periodic_voronoi([n] * 3, grains, rng)
+    rot = grain_rotations(grains, rng)
+    inclusion, centres = periodic_spheres([n] * 3, 8.0, 0.10, rng)
+    c11, c12, c44 = 250e3, 150e3, 120e3
+    k_v = (c11 + 2 * c12) / 3
+    mu_v = (c11 - c12 + 3 * c44) / 5
+    k, mu = 10 * k_v, 10 * mu_v
+    e = 9 * k * mu / (3 * k + mu)
+    m = Material(dimensions=[n] * 3).create_fields(
+        {"grain": grain_of_point, "phase": inclusion.astype(np.int64)})
+    m = m.create_regional_fields("grain", {"grain": np.arange(grains),
+                                           "orientation": Orientation(rot, "p")})
+    table = pd.DataFrame({"phase": [0, 1], "constitutive_model": [
+        serving.config3_evp(), Elastic(Order4SymmetricTensor.from_isotropic_constants(e, mu))]})
+    m = m.create_regional_fields("phase", table)
+    ls = LoadSchedule.from_constant_uniaxial_strain_rate(1e-3, "z")
+    res = {}
+    for prec in ("mixed", "fp64"):
+        print(f"[bench] 256^3 two-phase: {prec}", file=sys.stderr, flush=True)
+        res[prec] = _timed_run(m, ls, None, prec, steps * dt, dt, phase_label="phase")
+    (sm, km, tm), (sd, kd, td) = res["mixed"], res["fp64"]
+    vim, vid = n ** 3 * km / tm, n ** 3 * kd / td
+    return {
+        "workload": f"{n}^3 two-phase EVP, {grains} grains, {len(centres)} spheres r 8 "
+                    f"({inclusion.mean():.3f} vf), uniaxial z, {steps} x {dt} s",
+        "max_rel_stress_error": float(np.abs(sm - sd).max() / np.abs(sd).max()),
+        "krylov_mixed": int(km), "krylov_fp64": int(kd),
+        "vi_per_s_mixed": vim, "vi_per_s_fp64": vid,
+        "ms_per_step_mixed": 1e3 * tm / (steps - 1), "ms_per_step_fp64": 1e3 * td / (steps - 1),
+        "roofline_frac_mixed": vim * bytes_per_vi(n, "mixed") / (HBM_PEAK_GBS * 1e9),
+    }
+
+
+def workload(n, grains, seed=SEED):
     import numpy as np
 
     from materialite_amd.microstructure import grain_rotations, periodic_voronoi
 
-    rng = np.random.default_rng(SEED)
+    rng = np.random.default_rng(seed)
     grain_of_point, _ = periodic_voronoi([n, n, n], grains, rng)
     rot = grain_rotations(grains, rng)
     return grain_of_point, rot
 
 
-def build_material(n, grains):
+def build_material(n, grains, seed=SEED):
     import numpy as np
 
     from materialite_amd import Material, Orientation
 
-    grain_of_point, rot = workload(n, grains)
+    grain_of_point, rot = workload(n, grains, seed)
     m = Material(dimensions=[n, n, n]).create_fields({"grain": grain_of_point})
     return m.create_regional_fields(
         "grain", {"grain": np.arange(grains), "orientation": Orientation(rot, "p")})
@@ -180,7 +293,8 @@ def time_increments(material, precision, warmup, steps, profile):
 
     if warmup == 0:
         raise SystemExit("--warmup must be >= 1 (the first increment builds the solver state)")
-    model.run(material, increment_callback=cb)
+    out = model.run(material, increment_callback=cb)
+    rec["stress"] = out.extract("stress").components
     if profile:
         rec["profile"] = _native.profile_read()
         _native.profile_enable(False)
@@ -385,6 +499,8 @@ def main():
     if args.compare_fp64 and args.precision != "fp64":
         rec = time_increments(material, "fp64", args.warmup, args.steps, profile=False)
         fp64 = {
+            "max_rel_stress_error_of_mixed": float(
+                abs(main_rec["stress"] - rec["stress"]).max() / abs(rec["stress"]).max()),
             "value": npts * rec["krylov"] / rec["seconds"],
             "krylov_iterations": rec["krylov"],
             "seconds": rec["seconds"],
@@ -392,6 +508,8 @@ def main():
         }
     stream = (serving_stream(args.stream_requests, args.stream_rate, args.stream_slots)
               if args.stream_requests > 0 else None)
+    study = precision_study() if args.precision_study else None
+    large = large_two_phase() if args.large else None
     cpu = cpu_baseline(n, args.grains, args.cpu_iterations) if args.cpu_baseline else None
     line = {
         "metric": "voxel-iters/s at 128³ EVP, FP32-refined vs FP64; p50/p99 solve latency mixed",
@@ -428,6 +546,9 @@ def main():
         "kernels_launches": {k: int(v[1]) for k, v in prof.items()},
         "fp64_form": fp64,
         "serving": stream,
+        "precision_study_64": study,
+        "two_phase_256": large,
+        "vi_roofline_frac": vi * bytes_per_vi(n, args.precision) / (HBM_PEAK_GBS * 1e9),
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))

@@ -105,3 +105,23 @@ def test_solver_argument_errors():
     with pytest.raises(TypeError):
         ElasticViscoplastic(Order4SymmetricTensor.identity(), SlipSystem.octahedral(), 1.0, 10.0, 1.0,
                             lambda *a: None, {})
+
+
+def test_periodic_spheres_matches_brute_force():
+    from materialite_amd.microstructure import periodic_spheres
+
+    dims = np.array([20, 24, 18])
+    mask, centres = periodic_spheres(dims, 3.5, 0.1, np.random.default_rng(3))
+    g = np.stack(np.meshgrid(*[np.arange(d) for d in dims], indexing="ij"), -1).reshape(-1, 3)
+    ref = np.zeros(len(g), bool)
+    for c in centres:
+        d = np.abs(g - c)
+        d = np.minimum(d, dims - d)
+        ref |= np.linalg.norm(d, axis=1) <= 3.5
+    assert np.array_equal(mask, ref)
+    assert mask.mean() >= 0.1
+    # non-overlapping: periodic centre distances >= 2 r
+    for i in range(len(centres)):
+        d = np.abs(centres[i + 1:] - centres[i])
+        d = np.minimum(d, dims - d)
+        assert np.all(np.linalg.norm(d, axis=1) >= 7.0)
