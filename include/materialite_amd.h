@@ -46,7 +46,9 @@ enum { MTL_FIELD_STRESS = 0,            /* 6 per point */
        MTL_FIELD_SLIP_RESISTANCE = 3,   /* nslip */
        MTL_FIELD_SLIP_SHEAR = 4,        /* nslip */
        MTL_FIELD_ACCUMULATED_SLIP = 5,  /* 1 */
-       MTL_FIELD_TANGENT = 6            /* 36 (full Mandel 6x6) */ };
+       MTL_FIELD_TANGENT = 6,           /* 36 (full Mandel 6x6) */
+       MTL_FIELD_EQ_PLASTIC_STRAIN = 7, /* 1, J2 trial state */
+       MTL_FIELD_YIELD_STRESS = 8       /* 1, J2 trial state */ };
 
 const char* mtl_last_error(void);
 
@@ -65,6 +67,16 @@ int mtl_solver_add_evp(mtl_solver* h, const double stiffness[36], int num_slip,
                        const double* schmid, double reference_slip_rate,
                        double rate_exponent, double slip_resistance, int hardening,
                        const double hardening_props[4], int* model_id);
+
+/* IsotropicElasticPlastic (J2 radial return with isotropic hardening):
+ * small_strain_fft/isotropic_elastic_plastic.py:17-192.  stiffness: the
+ * model's isotropic Mandel 6x6 (from_isotropic_constants, tensor.py:1452-1467);
+ * bulk modulus as computed at isotropic_elastic_plastic.py:29-31; hardening
+ * MTL_HARDENING_* with the props of mtl_solver_add_evp, applied to the
+ * equivalent plastic strain. */
+int mtl_solver_add_j2(mtl_solver* h, const double stiffness[36], double shear_modulus,
+                      double bulk_modulus, double yield_stress, int hardening,
+                      const double hardening_props[4], int* model_id);
 
 /* Microstructure: per-point rotation matrices (N x 9, v_crystal = R v_specimen,
  * tensor.py:1855-1896) and per-point model ids (N int32; Multiphase dispatch,

@@ -32,6 +32,8 @@ FIELD = {
     "slip_system_shear_strains": 4,
     "accumulated_slip": 5,
     "tangent": 6,
+    "eq_plastic_strains": 7,
+    "yield_stresses": 8,
 }
 
 _D6 = ctypes.c_double * 6
@@ -78,6 +80,9 @@ EXPORTS = {
                                           ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
                                           ctypes.POINTER(ctypes.c_int)]),
+    "mtl_solver_add_j2": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
+                                         ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "mtl_solver_set_microstructure": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                                      ctypes.c_void_p]),
     "mtl_solver_initialize": (ctypes.c_int, [ctypes.c_void_p]),
@@ -198,6 +203,16 @@ class NativeSolver:
             self._h, _ptr(c), s.shape[0], _ptr(s), float(reference_slip_rate),
             float(rate_exponent), float(slip_resistance), int(hardening), _ptr(p),
             ctypes.byref(mid)))
+        return mid.value
+
+    def add_j2(self, stiffness, shear_modulus, bulk_modulus, yield_stress, hardening, props):
+        c = np.ascontiguousarray(stiffness, dtype=np.float64).reshape(36)
+        p = np.zeros(4)
+        p[: len(props)] = props
+        mid = ctypes.c_int()
+        check(self._lib.mtl_solver_add_j2(self._h, _ptr(c), float(shear_modulus),
+                                          float(bulk_modulus), float(yield_stress),
+                                          int(hardening), _ptr(p), ctypes.byref(mid)))
         return mid.value
 
     def set_microstructure(self, rotations, model_of_point):

@@ -857,12 +857,345 @@ class EvpModel final : public Model<K> {
   EvpParams<NS>* qh_ = nullptr;  // pinned staging copy
 };
 
+// ------------------------------------------------------------------ J2 plasticity
+// IsotropicElasticPlastic (isotropic_elastic_plastic.py:17-192): elastic
+// predictor s = 2G dev(eps - ep_old), p = K tr(eps - ep_old); where the von
+// Mises predictor exceeds the committed yield stress, the equivalent plastic
+// strain increment follows the fixed-point iteration
+//   r = max(mises - 3G de - Y, 0),  de <- de + r / (3G),  Y <- h(de / dt)
+// for every point together until |de_k - de_{k-1}| < 1e-7 at every point
+// (at most 100 iterations).  The iteration count is global, so the device
+// first runs every point's sequence once, OR-ing into a 128-bit mask the
+// iterations at which the point has not converged (k_j2_scan); the first
+// iteration with no bit set is the count the reference stops at, and
+// k_j2_finish replays each point to exactly that iteration.
+constexpr int kJ2MaxIterations = 100;    // isotropic_elastic_plastic.py:80
+constexpr double kJ2Tolerance = 1.0e-7;  // :79
+
+struct J2Params {
+  double C[21];  // isotropic stiffness, packed Mandel
+  double G, K, dt;
+  double hp[4];
+  int hard;
+  int pad;
+};
+
+struct J2Control {
+  unsigned int mask[4];  // bit k-1: some point not converged at iteration k
+  int plastic;           // some point's predictor exceeds its yield stress
+  int pad[3];
+};
+
+struct J2Buffers {
+  double* ep_old;  // [6][n] committed plastic strain
+  double* eq_old;  // [n] committed equivalent plastic strain
+  double* y_old;   // [n] committed yield stress
+  double* ep_pend; // outputs of the call
+  double* eq_pend;
+  double* y_pend;
+};
+
+// hardening law of one point: h(eq_old, y_old, rate = de / dt, dt), the
+// reference's Scalar branch (hardening_laws.py:18-90, slip increment dt |rate|)
+__device__ __forceinline__ double j2_harden(const J2Params& q, double eq_old, double y_old,
+                                            double de) {
+  const double d = q.dt * fabs(de / q.dt);
+  if (q.hard == kLinear) return y_old + q.hp[0] * d;
+  if (q.hard == kVoce) {
+    const double theta0 = q.hp[0], theta1 = q.hp[1], tau1 = q.hp[2];
+    const double ratio = fabs(theta0 / tau1);
+    const double e0 = exp(-ratio * eq_old);
+    const double ed = exp(-ratio * d);
+    return y_old + theta1 * d - (ratio * tau1 - theta1) / ratio * e0 * (ed - 1.0) -
+           theta1 / ratio * e0 * (ed * (1.0 + ratio * (eq_old + d)) - 1.0 - ratio * eq_old);
+  }
+  if (q.hard == kArmstrongFrederick) {
+    const double h = q.hp[0], r = q.hp[1];
+    return y_old + (h - r * y_old) * d / (1.0 + r * d);
+  }
+  return y_old;
+}
+
+struct J2Predictor {
+  double s[6];   // deviatoric predictor stress
+  double mean;   // K tr(elastic predictor strain)
+  double mises;  // sqrt(3/2) |s|
+};
+
+__device__ __forceinline__ J2Predictor j2_predict(const J2Params& q, const double* eps,
+                                                  long long npts, long long p,
+                                                  const double* ep_old, long long n, int i) {
+  J2Predictor r;
+  double ee[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) ee[c] = eps[c * npts + p] - ep_old[c * n + i];
+  const double tr = ee[0] + ee[1] + ee[2];
+  const double vol = 1.0 / 3.0 * tr;
+  double nrm = 0.0;
+#pragma unroll
+  for (int c = 0; c < 6; ++c) {
+    const double dv = c < 3 ? ee[c] - vol : ee[c];
+    r.s[c] = 2.0 * q.G * dv;
+    nrm += r.s[c] * r.s[c];
+  }
+  r.mean = q.K * tr;
+  r.mises = sqrt(1.5) * sqrt(nrm);
+  return r;
+}
+
+// one fixed-point iteration of a point (isotropic_elastic_plastic.py:108-126)
+__device__ __forceinline__ void j2_step(const J2Params& q, double mises, double eq_old,
+                                        double y_old, double& de, double& y, double& diff) {
+  double r = mises - 3.0 * q.G * de - y;
+  if (r <= 0.0) r = 0.0;
+  const double prev = de;
+  de = de - (-1.0 / (3.0 * q.G)) * r;
+  y = j2_harden(q, eq_old, y_old, de);
+  diff = de - prev;
+}
+
+__global__ void __launch_bounds__(kCBT) k_j2_scan(int n, const int* idx, long long npts,
+                                                   const J2Params* qg, const double* eps,
+                                                   J2Buffers B, J2Control* ctl) {
+  __shared__ J2Params q;
+  __shared__ unsigned int smask[4];
+  __shared__ int splastic;
+  load_params(&q, qg);
+  if (threadIdx.x < 4) smask[threadIdx.x] = 0u;
+  if (threadIdx.x == 0) splastic = 0;
+  __syncthreads();
+  const int i = blockIdx.x * kCBT + threadIdx.x;
+  if (i < n) {
+    const long long p = idx[i];
+    const J2Predictor pr = j2_predict(q, eps, npts, p, B.ep_old, n, i);
+    const double y_old = B.y_old[i], eq_old = B.eq_old[i];
+    if (!(pr.mises <= y_old)) {
+      atomicOr(&splastic, 1);
+      unsigned int m[4] = {0u, 0u, 0u, 0u};
+      double de = 0.0, y = y_old, diff = 0.0;
+      for (int k = 1; k <= kJ2MaxIterations; ++k) {
+        j2_step(q, pr.mises, eq_old, y_old, de, y, diff);
+        if (!(fabs(diff) < kJ2Tolerance)) m[(k - 1) >> 5] |= 1u << ((k - 1) & 31);
+        else if (diff == 0.0) break;  // fixed point: every later iterate is identical
+      }
+#pragma unroll
+      for (int w = 0; w < 4; ++w)
+        if (m[w]) atomicOr(&smask[w], m[w]);
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x < 4 && smask[threadIdx.x]) atomicOr(&ctl->mask[threadIdx.x], smask[threadIdx.x]);
+  if (threadIdx.x == 0 && splastic) atomicOr(&ctl->plastic, 1);
+}
+
+// Stress, tangent and (when `plastic`) pending state after `iters` iterations
+// (isotropic_elastic_plastic.py:89-168); all-elastic calls return the
+// predictor and the stiffness and leave the state alone (:89-95).
+template <typename K>
+__global__ void __launch_bounds__(kCBT) k_j2_finish(int n, const int* idx, long long npts,
+                                                     const J2Params* qg, const double* eps,
+                                                     int iters, int plastic, J2Buffers B,
+                                                     double* sig, K* tangent) {
+  __shared__ J2Params q;
+  load_params(&q, qg);
+  __syncthreads();
+  const int i = blockIdx.x * kCBT + threadIdx.x;
+  if (i >= n) return;
+  const long long p = idx[i];
+  const long long ln = n;
+  const J2Predictor pr = j2_predict(q, eps, npts, p, B.ep_old, n, i);
+  const double y_old = B.y_old[i], eq_old = B.eq_old[i];
+  const bool elastic = pr.mises <= y_old;
+  double Ds[21];
+#pragma unroll
+  for (int k = 0; k < 21; ++k) Ds[k] = q.C[k];
+  if (!plastic) {
+#pragma unroll
+    for (int c = 0; c < 6; ++c) sig[c * npts + p] = pr.s[c] + (c < 3 ? pr.mean : 0.0);
+    store_tangent(tangent, npts, p, Ds);
+    return;
+  }
+  double de = 0.0, y = y_old, diff = 0.0;
+  for (int k = 1; k <= iters; ++k) j2_step(q, pr.mises, eq_old, y_old, de, y, diff);
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+    B.ep_pend[c * ln + i] = B.ep_old[c * ln + i] + 1.5 * (pr.s[c] / pr.mises) * de;
+  B.eq_pend[i] = eq_old + de;
+  B.y_pend[i] = y;
+  const double ratio = 3.0 * q.G / pr.mises;
+#pragma unroll
+  for (int c = 0; c < 6; ++c)
+    sig[c * npts + p] = (1.0 - ratio * de) * pr.s[c] + (c < 3 ? pr.mean : 0.0);
+  if (!elastic) {
+    const double term1 = 2.0 * q.G * ratio * de;
+    const double term2 = ratio * ratio * (de / pr.mises - 1.0 / (3.0 * q.G));
+#pragma unroll
+    for (int a = 0; a < 6; ++a)
+#pragma unroll
+      for (int b = a; b < 6; ++b) {
+        const double idev = (a == b ? 1.0 : 0.0) - (a < 3 && b < 3 ? 1.0 / 3.0 : 0.0);
+        Ds[sym6(a, b)] = q.C[sym6(a, b)] - term1 * idev + term2 * (pr.s[a] * pr.s[b]);
+      }
+  }
+  store_tangent(tangent, npts, p, Ds);
+}
+
+// the stiffness as every point's tangent (isotropic: the same in every frame)
+template <typename K>
+__global__ void __launch_bounds__(kCBT) k_const_tangent(int n, const int* idx, long long npts,
+                                                         Stiff21 C, K* tangent) {
+  const int i = blockIdx.x * kCBT + threadIdx.x;
+  if (i >= n) return;
+  store_tangent(tangent, npts, idx[i], C.c);
+}
+
+__global__ void k_j2_ctrl_reset(J2Control* c) {
+  int* w = reinterpret_cast<int*>(c);
+  for (int i = threadIdx.x; i < int(sizeof(J2Control) / sizeof(int)); i += blockDim.x) w[i] = 0;
+}
+
+template <typename K>
+class J2Model final : public Model<K> {
+ public:
+  J2Model(const ModelDesc& d, const std::vector<int>& pts, long long npts) : d_(d) {
+    this->n_ = int(pts.size());
+    this->npts_ = npts;
+    this->host_idx_ = pts;
+    this->idx_ = upload_idx(pts);
+    const Stiff21 c = pack21(d.C);
+    for (int k = 0; k < 21; ++k) q_.C[k] = c.c[k];
+    q_.G = d.shear_modulus;
+    q_.K = d.bulk_modulus;
+    q_.hard = d.hardening;
+    for (int k = 0; k < 4; ++k) q_.hp[k] = d.hp[k];
+    const long long n = std::max(this->n_, 1);
+    auto alloc = [&](double*& ptr, long long cnt) { MTL_HIP(hipMalloc(&ptr, cnt * sizeof(double))); };
+    alloc(B_.ep_old, 6 * n);
+    alloc(B_.eq_old, n);
+    alloc(B_.y_old, n);
+    alloc(B_.ep_pend, 6 * n);
+    alloc(B_.eq_pend, n);
+    alloc(B_.y_pend, n);
+    alloc(ep_tr_, 6 * n);
+    alloc(eq_tr_, n);
+    alloc(y_tr_, n);
+    MTL_HIP(hipMalloc(&ctl_, sizeof(J2Control)));
+    MTL_HIP(hipMalloc(&qd_, sizeof(J2Params)));
+    MTL_HIP(hipHostMalloc(&qh_, sizeof(J2Params), hipHostMallocDefault));
+    MTL_HIP(hipHostMalloc(&hctl_, sizeof(J2Control), hipHostMallocDefault));
+  }
+  ~J2Model() override {
+    for (double* p : {B_.ep_old, B_.eq_old, B_.y_old, B_.ep_pend, B_.eq_pend, B_.y_pend, ep_tr_,
+                      eq_tr_, y_tr_})
+      if (p) (void)hipFree(p);
+    if (ctl_) (void)hipFree(ctl_);
+    if (qd_) (void)hipFree(qd_);
+    if (qh_) (void)hipHostFree(qh_);
+    if (hctl_) (void)hipHostFree(hctl_);
+    (void)hipFree(this->idx_);
+  }
+
+  void init_tangent(const double* rot9, K* tangent, hipStream_t s) override {
+    // isotropic_elastic_plastic.py:55-75: zero plastic state, yield stress Y0
+    const long long n = this->n_;
+    if (n == 0) return;
+    for (double* p : {B_.ep_old, ep_tr_}) fill(p, 6 * n, 0.0, s);
+    for (double* p : {B_.eq_old, eq_tr_}) fill(p, n, 0.0, s);
+    for (double* p : {B_.y_old, y_tr_}) fill(p, n, d_.g0, s);
+    hipLaunchKernelGGL(k_const_tangent<K>, dim3(ceil_div(this->n_, kCBT)), dim3(kCBT), 0, s,
+                       this->n_, this->idx_, this->npts_, pack21(d_.C), tangent);
+    MTL_LAUNCH_CHECK();
+  }
+
+  int calculate(const double* rot9, const double* eps, const double*, double dt, double* sig,
+                K* tangent, hipStream_t s) override {
+    const int n = this->n_;
+    if (n == 0) return 1;
+    q_.dt = dt;
+    *qh_ = q_;
+    MTL_HIP(hipMemcpyAsync(qd_, qh_, sizeof(J2Params), hipMemcpyHostToDevice, s));
+    const int blocks = ceil_div(n, kCBT);
+    hipLaunchKernelGGL(k_j2_ctrl_reset, dim3(1), dim3(64), 0, s, ctl_);
+    MTL_LAUNCH_CHECK();
+    profiler().begin(s);
+    hipLaunchKernelGGL(k_j2_scan, dim3(blocks), dim3(kCBT), 0, s, n, this->idx_, this->npts_, qd_,
+                       eps, B_, ctl_);
+    MTL_LAUNCH_CHECK();
+    profiler().end(kKElastic, s);
+    MTL_HIP(hipMemcpyAsync(hctl_, ctl_, sizeof(J2Control), hipMemcpyDeviceToHost, s));
+    MTL_HIP(hipStreamSynchronize(s));
+    int iters = 1;
+    if (hctl_->plastic) {
+      iters = 0;
+      for (int k = 1; k <= kJ2MaxIterations; ++k)
+        if (!((hctl_->mask[(k - 1) >> 5] >> ((k - 1) & 31)) & 1u)) {
+          iters = k;
+          break;
+        }
+      if (iters == 0) return 0;  // "Too many iterations in material loop" (:128-130)
+    }
+    profiler().begin(s);
+    hipLaunchKernelGGL(k_j2_finish<K>, dim3(blocks), dim3(kCBT), 0, s, n, this->idx_, this->npts_,
+                       qd_, eps, iters, hctl_->plastic, B_, sig, tangent);
+    MTL_LAUNCH_CHECK();
+    profiler().end(kKElastic, s);
+    if (hctl_->plastic) {
+      // the call's state becomes the trial state (isotropic_elastic_plastic.py:131-137)
+      std::swap(B_.ep_pend, ep_tr_);
+      std::swap(B_.eq_pend, eq_tr_);
+      std::swap(B_.y_pend, y_tr_);
+    }
+    return iters;
+  }
+
+  void commit(const double*, hipStream_t s) override {
+    // isotropic_elastic_plastic.py:170-179
+    const size_t n = size_t(this->n_);
+    if (n == 0) return;
+    MTL_HIP(hipMemcpyAsync(B_.ep_old, ep_tr_, 6 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    MTL_HIP(hipMemcpyAsync(B_.eq_old, eq_tr_, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    MTL_HIP(hipMemcpyAsync(B_.y_old, y_tr_, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  }
+
+  bool field(int which, const double*, std::vector<double>& out, int& ncomp,
+             hipStream_t s) override {
+    const int n = this->n_;
+    auto fetch = [&](const double* src, int nc) {
+      std::vector<double> soa(size_t(nc) * n);
+      if (n) MTL_HIP(hipMemcpyAsync(soa.data(), src, soa.size() * sizeof(double),
+                                    hipMemcpyDeviceToHost, s));
+      MTL_HIP(hipStreamSynchronize(s));
+      out.resize(size_t(nc) * n);
+      for (int i = 0; i < n; ++i)
+        for (int c = 0; c < nc; ++c) out[size_t(i) * nc + c] = soa[size_t(c) * n + i];
+      ncomp = nc;
+    };
+    switch (which) {
+      case kFieldPlasticStrain: fetch(ep_tr_, 6); return true;
+      case kFieldEqPlasticStrain: fetch(eq_tr_, 1); return true;
+      case kFieldYieldStress: fetch(y_tr_, 1); return true;
+      default: return false;
+    }
+  }
+
+ private:
+  ModelDesc d_;
+  J2Params q_{};
+  J2Buffers B_{};
+  double *ep_tr_ = nullptr, *eq_tr_ = nullptr, *y_tr_ = nullptr;
+  J2Control* ctl_ = nullptr;
+  J2Control* hctl_ = nullptr;
+  J2Params* qd_ = nullptr;
+  J2Params* qh_ = nullptr;
+};
+
 }  // namespace
 
 template <typename K>
 std::unique_ptr<Model<K>> make_model(const ModelDesc& d, const std::vector<int>& points,
                                      long long npts) {
   if (d.kind == 0) return std::make_unique<ElasticModel<K>>(d, points, npts);
+  if (d.kind == 2) return std::make_unique<J2Model<K>>(d, points, npts);
   switch (d.nslip) {
     case 12: return std::make_unique<EvpModel<K, 12>>(d, points, npts);
     case 1: return std::make_unique<EvpModel<K, 1>>(d, points, npts);

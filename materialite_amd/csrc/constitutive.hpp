@@ -14,14 +14,16 @@ constexpr int kEvpMaxIterations = 60;  // elastic_viscoplastic.py:93
 constexpr double kEvpTolerance = 2.0e-5;  // elastic_viscoplastic.py:92
 
 struct ModelDesc {
-  int kind = 0;  // 0 elastic, 1 elasto-viscoplastic
+  int kind = 0;  // 0 elastic, 1 elasto-viscoplastic, 2 isotropic elastic-plastic (J2)
   double C[36] = {};  // crystal-frame stiffness, Mandel
   // elasto-viscoplastic
   int nslip = 0;
   std::vector<double> schmid;  // nslip*6, Mandel sym(d (x) n)
-  double gamma0 = 0, n = 0, g0 = 0;
+  double gamma0 = 0, n = 0, g0 = 0;  // g0: initial slip resistance / J2 yield stress
   int hardening = kPerfectPlasticity;
   double hp[4] = {};
+  // J2: shear and bulk moduli
+  double shear_modulus = 0, bulk_modulus = 0;
 };
 
 // Device state and kernels of one model over its subset of points.
@@ -60,6 +62,8 @@ enum FieldId {
   kFieldSlipShear = 4,
   kFieldAccumulatedSlip = 5,
   kFieldTangent = 6,
+  kFieldEqPlasticStrain = 7,
+  kFieldYieldStress = 8,
 };
 
 }  // namespace mtl

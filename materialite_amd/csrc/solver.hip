@@ -677,6 +677,25 @@ int mtl_solver_add_evp(mtl_solver* h, const double stiffness[36], int num_slip,
   });
 }
 
+int mtl_solver_add_j2(mtl_solver* h, const double stiffness[36], double shear_modulus,
+                      double bulk_modulus, double yield_stress, int hardening,
+                      const double hardening_props[4], int* model_id) {
+  if (!h || !stiffness || !model_id) return bad("null argument");
+  if (hardening < 0 || hardening > 3) return bad("unknown hardening law");
+  if (!(shear_modulus > 0.0)) return bad("J2 model needs a positive shear modulus");
+  return guarded([&] {
+    mtl::ModelDesc d;
+    d.kind = 2;
+    std::memcpy(d.C, stiffness, sizeof(d.C));
+    d.shear_modulus = shear_modulus;
+    d.bulk_modulus = bulk_modulus;
+    d.g0 = yield_stress;
+    d.hardening = hardening;
+    if (hardening_props) std::memcpy(d.hp, hardening_props, sizeof(d.hp));
+    *model_id = h->impl->add_model(d);
+  });
+}
+
 int mtl_solver_set_microstructure(mtl_solver* h, const double* rotations,
                                   const int32_t* model_of_point) {
   if (!h || !rotations) return bad("null argument");

@@ -22,6 +22,7 @@ from ..._native import NativeSolver, StepParams
 from ...tensor import Order2SymmetricTensor, Scalar
 from .elastic import Elastic
 from .elastic_viscoplastic import ElasticViscoplastic
+from .isotropic_elastic_plastic import IsotropicElasticPlastic
 from .multiphase import Multiphase
 
 
@@ -81,9 +82,14 @@ class SmallStrainFFT:
                                   float(model.reference_slip_rate.components),
                                   float(model.rate_exponent.components),
                                   float(model.initial_slip_resistance), kind, props)
+        if isinstance(model, IsotropicElasticPlastic):
+            kind, props = model.kernel_hardening()
+            return native.add_j2(model.stiffness.components, model.shear_modulus,
+                                 model.bulk_modulus, float(model.yield_stress.components),
+                                 kind, props)
         raise TypeError(
             f"constitutive model {type(model).__name__} has no device implementation "
-            "(supported: Elastic, ElasticViscoplastic)")
+            "(supported: Elastic, ElasticViscoplastic, IsotropicElasticPlastic)")
 
     def _new_time_increment(self, time, time_increment, next_output_time, max_constit_iters):
         """small_strain_fft.py:266-279."""
@@ -107,6 +113,8 @@ class SmallStrainFFT:
             data = native.get_field(v)
             if v == "plastic_strains":
                 out[v] = Order2SymmetricTensor(data, "p")
+            elif v in ("eq_plastic_strains", "yield_stresses"):
+                out[v] = Scalar(data[:, 0], "p")
             else:
                 out[v] = Scalar(data, "ps")
         return out
@@ -186,10 +194,16 @@ class SmallStrainFFT:
                     old_max_strain_increment = max_strain_increment
                     time_step_id += 1
                     time += time_increment
+                    # the reference collects outputs, advances the output time and then
+                    # sizes the next increment (small_strain_fft.py:221-236); the device
+                    # commit (state, and fluctuation scaled by the new increment) comes
+                    # first here, which leaves the collected fields unchanged
+                    output_now = time >= (next_output_time - time_tolerance)
+                    upcoming = output_times[time_idx + 1] if output_now else next_output_time
                     new_time_increment = self._new_time_increment(
-                        time, time_increment, next_output_time, rep.max_constitutive_iterations)
+                        time, time_increment, upcoming, rep.max_constitutive_iterations)
                     native.commit(new_time_increment / time_increment)
-                    if time >= (next_output_time - time_tolerance):
+                    if output_now:
                         outputs = self._outputs(native, models, output_variables)
                         outputs.update({"stress": Order2SymmetricTensor(native.get_field("stress"), "p"),
                                         "strain": Order2SymmetricTensor(native.get_field("strain"), "p")})

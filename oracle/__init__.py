@@ -19,7 +19,10 @@ Pinning (how each function is checked, see ``tests/test_oracle_*.py``):
 * ``minres``        iterate-for-iterate against ``scipy.sparse.linalg.minres``.
 * ``constitutive``  closed-form hardening identities; single-crystal closed forms;
                     end-to-end reference goldens (EVP 8^3: 361.276725, 541.043871,
-                    539.345343, 442.608112, 426.923816/428.328875).
+                    539.345343, 442.608112, 426.923816/428.328875; J2 8^3: 150.0,
+                    150.3980086, 171.694205 and the reference's step relations
+                    between axial stress and equivalent plastic strain; J2
+                    two-phase 150.3980086).
 * ``solver``        end-to-end reference goldens (elastic: 6.5890252, 6.0,
                     4.547568053, homogeneous closed forms, periodic-shift
                     invariance).

@@ -8,6 +8,9 @@ Restates, on plain arrays (points p, Mandel components m, slip systems s):
   least half of the points have converged, consistent tangent J^-1 C)
 * hardening laws           ``small_strain_fft/hardening_laws.py:18-90``
 * ``Multiphase``           ``small_strain_fft/multiphase.py:18-95``
+* ``IsotropicElasticPlastic`` ``small_strain_fft/isotropic_elastic_plastic.py:17-192``
+  (J2 radial return with isotropic hardening, fixed-point iteration on the
+  equivalent plastic strain increment until every point has converged)
 """
 
 import logging
@@ -16,6 +19,7 @@ import numpy as np
 
 from .tensors import (
     directional_modulus,
+    isotropic_stiffness,
     order2_to_crystal,
     order2_to_specimen,
     order4_to_specimen,
@@ -238,3 +242,98 @@ class Multiphase:
     def update_state_variables(self):
         for m in self.models:
             m.update_state_variables()
+
+
+# ---------------------------------------------------------------- J2 plasticity
+MANDEL_IDENTITY = np.array([1.0, 1.0, 1.0, 0.0, 0.0, 0.0])
+
+
+class IsotropicElasticPlastic:
+    """J2 plasticity with isotropic hardening (isotropic_elastic_plastic.py:17-192).
+
+    The hardening law is one of the ``HARDENING`` functions applied per point
+    with a single "slip system" (equivalent plastic strain rate), as the
+    reference's laws fall back to for Scalars (hardening_laws.py:34-37).
+    """
+
+    TOLERANCE = 1.0e-7      # isotropic_elastic_plastic.py:79
+    MAX_ITERATIONS = 100    # :80
+
+    def __init__(self, modulus, shear_modulus, yield_stress, hardening, properties):
+        self.modulus = modulus
+        self.shear_modulus = shear_modulus
+        # isotropic_elastic_plastic.py:29-31
+        self.bulk_modulus = shear_modulus * modulus / (3 * (3 * shear_modulus - modulus))
+        self.stiffness = isotropic_stiffness(modulus, shear_modulus)
+        self.yield_stress = float(yield_stress)
+        self.hardening = HARDENING[hardening] if isinstance(hardening, str) else hardening
+        self.properties = properties
+
+    def initialize(self, rotations):
+        n = len(rotations)
+        self.ep = np.zeros((n, 6))
+        self.ep_old = np.zeros((n, 6))
+        self.eq = np.zeros(n)
+        self.eq_old = np.zeros(n)
+        self.y = np.full(n, self.yield_stress)
+        self.y_old = np.full(n, self.yield_stress)
+        return np.broadcast_to(self.stiffness, (n, 6, 6)).copy()
+
+    def _harden(self, rate, dt):
+        return self.hardening(self.properties, self.eq_old, self.y_old[:, None],
+                              rate[:, None], dt)[:, 0]
+
+    def calculate_stress_and_tangent(self, strain, guess_stress, dt):
+        """isotropic_elastic_plastic.py:78-168."""
+        g, k = self.shear_modulus, self.bulk_modulus
+        n = len(strain)
+        ee = strain - self.ep_old
+        tr = ee[:, 0] + ee[:, 1] + ee[:, 2]
+        dev = ee - (1 / 3 * tr)[:, None] * MANDEL_IDENTITY
+        s_pred = 2.0 * g * dev
+        mean = k * tr
+        mises = np.sqrt(1.5) * np.sqrt(np.einsum("pn,pn->p", s_pred, s_pred))
+        with np.errstate(divide="ignore", invalid="ignore"):
+            ratios = s_pred / mises[:, None]
+        jinv = -1.0 / (3.0 * g)
+        elastic = mises <= self.y_old
+        tangent = np.broadcast_to(self.stiffness, (n, 6, 6)).copy()
+        if np.all(elastic):
+            return s_pred + mean[:, None] * MANDEL_IDENTITY, tangent, 1
+        deps = np.zeros(n)
+        y = self.y_old.copy()
+        iteration = 0
+        while True:
+            iteration += 1
+            r = mises - 3.0 * g * deps - y
+            r[r <= 0.0] = 0.0
+            prev = deps.copy()
+            deps = deps - jinv * r
+            y = self._harden(deps / dt, dt)
+            converged = np.all(np.abs(deps - prev) < self.TOLERANCE)
+            if converged:
+                break
+            if iteration == self.MAX_ITERATIONS:
+                return None, None, False
+        with np.errstate(invalid="ignore"):
+            self.ep = self.ep_old + 1.5 * ratios * deps[:, None]
+        self.eq = self.eq_old + deps
+        self.y = y
+        ratio = 3.0 * g / mises
+        stress = (1.0 - ratio * deps)[:, None] * s_pred + mean[:, None] * MANDEL_IDENTITY
+        dev_identity = np.eye(6) - np.outer(MANDEL_IDENTITY, MANDEL_IDENTITY) / 3.0
+        term1 = 2.0 * g * ratio * deps
+        term2 = ratio ** 2 * (deps / mises - 1 / (3 * g))
+        d = (tangent - term1[:, None, None] * dev_identity
+             + term2[:, None, None] * np.einsum("pi,pj->pij", s_pred, s_pred))
+        d[elastic] = tangent[elastic]
+        return stress, d, iteration
+
+    def update_state_variables(self):
+        self.ep_old = self.ep.copy()
+        self.eq_old = self.eq.copy()
+        self.y_old = self.y.copy()
+
+    def outputs(self):
+        return {"plastic_strains": self.ep.copy(), "eq_plastic_strains": self.eq.copy(),
+                "yield_stresses": self.y.copy()}

@@ -56,3 +56,21 @@ def voronoi_8():
 def evp_model(hardening, props):
     schmid = T.schmid_mandel(*T.octahedral_slip_systems())
     return C.ElasticViscoplastic(TI_STIFFNESS, schmid, 1.0, 10.0, 300.0, hardening, props)
+
+
+# J2 plasticity cases of tests/test_isotropic_plasticity.py:19-140 (8^3, identity
+# orientation, sizes 3, uniaxial z, dt 1e-4): (hardening, properties, steps, golden)
+J2_CASES = {
+    "perfect_plasticity": ("perfect_plasticity", None, 15, 150.0),
+    "linear": ("linear", {"hardening_rate": 1000.0}, 15, 150.3980086),
+    "voce": ("voce", {"tau_1": 30.0, "theta_0": 10000.0, "theta_1": 100.0}, 50, 171.694205),
+}
+J2_MODULUS, J2_SHEAR_MODULUS, J2_YIELD = 150000.0, 60000.0, 150.0
+
+
+def box_phase_8():
+    """Phase of each point of the 8^3 grid with sizes 3 when a Box(max_corner=[2,2,2])
+    is inserted (tests/test_multiphase.py:13-38): coordinates i * 3/7 <= 2."""
+    i = np.arange(8) * 3.0 / 7.0
+    inside = i <= 2.0
+    return (inside[:, None, None] & inside[None, :, None] & inside[None, None, :]).ravel()

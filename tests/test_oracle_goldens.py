@@ -91,3 +91,52 @@ def test_evp_strain_bcs_vs_golden():
     assert_allclose(mean_norm(out["stress"]), 428.328875)
     assert_allclose(out["stress"].mean(axis=0)[2], 426.923816)
     assert_allclose(out["strain"].mean(axis=0), rate * 0.004, atol=1e-14)
+
+
+def _j2(name):
+    hard, props, steps, golden = F.J2_CASES[name]
+    return C.IsotropicElasticPlastic(F.J2_MODULUS, F.J2_SHEAR_MODULUS, F.J2_YIELD, hard, props), steps, golden
+
+
+def _j2_run(name, steps):
+    model = _j2(name)[0]
+    ls = LoadSchedule.from_constant_uniaxial_strain_rate(1.0, "z")
+    out = SmallStrainFFT(ls, end_time=steps * 1e-4, initial_time_increment=1e-4).run(
+        [8, 8, 8], np.broadcast_to(np.eye(3), (512, 3, 3)), model)
+    return out
+
+
+@pytest.mark.parametrize("name", ["perfect_plasticity", "linear", "voce"])
+def test_j2_vs_golden(name):
+    # tests/test_isotropic_plasticity.py:30-140
+    _, steps, golden = _j2(name)
+    out = _j2_run(name, steps)
+    assert_allclose(mean_norm(out["stress"]), golden)
+    axial = out["stress"][:, 2]
+    assert_allclose(axial, axial.mean())
+    # plastic flow is deviatoric
+    assert_allclose(out["plastic_strains"][:, :3].sum(axis=-1), 0.0, atol=1e-15)
+    # the reference's step relations: the axial stress of step k + 1 is the
+    # hardened yield stress of the equivalent plastic strain of step k
+    prev = _j2_run(name, steps - 1)["eq_plastic_strains"].mean()
+    if name == "perfect_plasticity":
+        assert_allclose(axial.mean(), 150.0)
+    elif name == "linear":
+        assert_allclose(axial.mean(), 1000.0 * prev + F.J2_YIELD)
+    else:
+        tau1, theta0, theta1 = 30.0, 10000.0, 100.0
+        voce = F.J2_YIELD + (tau1 + theta1 * prev) * (1 - np.exp(-prev * abs(theta0 / tau1)))
+        assert_allclose(axial.mean(), voce)
+
+
+def test_j2_multiphase_vs_golden():
+    # tests/test_multiphase.py:42-74: two identical linear-hardening phases
+    box = F.box_phase_8()
+    models = [_j2("linear")[0], _j2("linear")[0]]
+    model = C.Multiphase(models, [np.nonzero(~box)[0], np.nonzero(box)[0]], 512)
+    ls = LoadSchedule.from_constant_uniaxial_strain_rate(1.0, "z")
+    out = SmallStrainFFT(ls, end_time=15e-4, initial_time_increment=1e-4).run(
+        [8, 8, 8], np.broadcast_to(np.eye(3), (512, 3, 3)), model)
+    assert_allclose(mean_norm(out["stress"]), 150.3980086)
+    eq = np.concatenate([m.eq for m in models])
+    assert_allclose(eq, eq.mean())
