@@ -1196,14 +1196,22 @@ std::unique_ptr<Model<K>> make_model(const ModelDesc& d, const std::vector<int>&
                                      long long npts) {
   if (d.kind == 0) return std::make_unique<ElasticModel<K>>(d, points, npts);
   if (d.kind == 2) return std::make_unique<J2Model<K>>(d, points, npts);
+  // FCC octahedral (12); HCP basal / prismatic (3), pyramidal <a> (6),
+  // pyramidal <c+a> (12) and their concatenations (6, 9, 15, 18, 24)
   switch (d.nslip) {
     case 12: return std::make_unique<EvpModel<K, 12>>(d, points, npts);
     case 1: return std::make_unique<EvpModel<K, 1>>(d, points, npts);
+    case 2: return std::make_unique<EvpModel<K, 2>>(d, points, npts);
     case 3: return std::make_unique<EvpModel<K, 3>>(d, points, npts);
+    case 4: return std::make_unique<EvpModel<K, 4>>(d, points, npts);
     case 6: return std::make_unique<EvpModel<K, 6>>(d, points, npts);
+    case 9: return std::make_unique<EvpModel<K, 9>>(d, points, npts);
+    case 15: return std::make_unique<EvpModel<K, 15>>(d, points, npts);
+    case 18: return std::make_unique<EvpModel<K, 18>>(d, points, npts);
+    case 24: return std::make_unique<EvpModel<K, 24>>(d, points, npts);
     default:
       throw Error("elasto-viscoplastic model with " + std::to_string(d.nslip) +
-                  " slip systems is not built (supported: 1, 3, 6, 12)");
+                  " slip systems is not built (supported: 1, 2, 3, 4, 6, 9, 12, 15, 18, 24)");
   }
 }
 

@@ -204,6 +204,54 @@ def octahedral_slip_systems():
     return normals, directions
 
 
+def hexagonal_direction(uvtw, c_over_a=1.633):
+    """Miller-Bravais direction -> unit Cartesian (slip_system.py:158-164)."""
+    d = np.atleast_2d(np.asarray(uvtw, float))
+    x = d[:, 0] * 1.5
+    y = np.sqrt(3.0) * (0.5 * d[:, 0] + d[:, 1])
+    z = c_over_a * d[:, 3]
+    v = np.stack([x, y, z], axis=1)
+    return v / np.sqrt((v * v).sum(axis=1))[:, None]
+
+
+def hexagonal_plane(hkil, c_over_a=1.633):
+    """Miller-Bravais plane -> unit Cartesian normal (slip_system.py:167-173)."""
+    p = np.atleast_2d(np.asarray(hkil, float))
+    v = np.stack([p[:, 0], (p[:, 0] + 2.0 * p[:, 1]) / np.sqrt(3.0), p[:, 3] / c_over_a], axis=1)
+    return v / np.sqrt((v * v).sum(axis=1))[:, None]
+
+
+HCP_FAMILIES = {
+    # family: (plane indices per system, direction indices per system) (slip_system.py:67-137)
+    "basal": ([[0, 0, 0, 1]] * 3, [[2, -1, -1, 0], [-1, 2, -1, 0], [-1, -1, 2, 0]]),
+    "prismatic": ([[1, 0, -1, 0], [0, -1, 1, 0], [-1, 1, 0, 0]],
+                  [[-1, 2, -1, 0], [2, -1, -1, 0], [-1, -1, 2, 0]]),
+    "pyramidal_a": ([[1, 0, -1, 1], [0, -1, 1, 1], [-1, 1, 0, 1], [-1, 0, 1, 1],
+                     [0, 1, -1, 1], [1, -1, 0, 1]],
+                    [[-1, 2, -1, 0], [2, -1, -1, 0], [-1, -1, 2, 0], [-1, 2, -1, 0],
+                     [2, -1, -1, 0], [1, 1, -2, 0]]),
+    "pyramidal_ca": ([h for h in [[1, 0, -1, 1], [0, -1, 1, 1], [-1, 1, 0, 1], [-1, 0, 1, 1],
+                                  [0, 1, -1, 1], [1, -1, 0, 1]] for _ in (0, 1)],
+                     [[-1, -1, 2, 3], [-2, 1, 1, 3], [1, 1, -2, 3], [-1, 2, -1, 3],
+                      [2, -1, -1, 3], [1, -2, 1, 3], [2, -1, -1, 3], [1, 1, -2, 3],
+                      [-1, -1, 2, 3], [1, -2, 1, 3], [-2, 1, 1, 3], [-1, 2, -1, 3]]),
+}
+
+
+def hcp_slip_systems(families, c_over_a=1.633):
+    """Unit normals and directions of the concatenated HCP families."""
+    normals = np.concatenate([hexagonal_plane(HCP_FAMILIES[f][0], c_over_a) for f in families])
+    directions = np.concatenate(
+        [hexagonal_direction(HCP_FAMILIES[f][1], c_over_a) for f in families])
+    return normals, directions
+
+
+def max_schmid_factor(normals, directions, load):
+    """max_s |(d_s (x) n_s) : l (x) l| for a unit load direction l."""
+    l = np.asarray(load, float) / np.linalg.norm(load)
+    return np.max(np.abs((directions @ l) * (normals @ l)))
+
+
 def schmid_mandel(normals, directions):
     """sym(d (x) n) per system as Mandel 6-vectors  (slip_system.py:139-141)."""
     p = np.einsum("si,sj->sij", directions, normals)

@@ -125,3 +125,18 @@ def test_periodic_spheres_matches_brute_force():
         d = np.abs(centres[i + 1:] - centres[i])
         d = np.minimum(d, dims - d)
         assert np.all(np.linalg.norm(d, axis=1) >= 7.0)
+
+
+def test_hcp_slip_systems_match_oracle():
+    for fam, ca in [("basal", 1.633), ("prismatic", 1.633), ("pyramidal_a", 1.5), ("pyramidal_ca", 1.5)]:
+        ss = getattr(SlipSystem, fam)() if fam in ("basal", "prismatic") else getattr(SlipSystem, fam)(c_over_a=ca)
+        n, d = T.hcp_slip_systems([fam], ca)
+        assert_allclose(ss.normal.components, n, atol=1e-15)
+        assert_allclose(ss.direction.components, d, atol=1e-15)
+    both = SlipSystem.basal().concatenate(SlipSystem.prismatic())
+    assert len(both) == 6
+    from materialite_amd.tensor import Vector
+    assert_allclose(SlipSystem.pyramidal_ca(c_over_a=1.5).max_schmid_factor(Vector([1.0, 1.0, 1.0])).components,
+                    (7 + 5 * np.sqrt(3)) / (12 * np.sqrt(13)))
+    assert_allclose(SlipSystem.octahedral().max_schmid_factor(Vector([1.0, 0.0, 0.0])).components,
+                    1 / np.sqrt(6))

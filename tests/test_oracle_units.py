@@ -174,3 +174,39 @@ def test_linear_and_af_increments():
         C.armstrong_frederick({"direct_hardening": 100.0, "dynamic_recovery": 2.0}, np.zeros(1), np.full((1, 2), 5.0), rates, 2.0),
         5.0 + (100.0 - 2.0 * 5.0) * d / (1 + 2.0 * d),
     )
+
+
+def test_hcp_slip_systems_known_answers():
+    # tests/test_slip_system.py:62-200 (c/a = 1.5 for the pyramidal families)
+    r3 = np.sqrt(3)
+    r3b2 = r3 / 2
+    n, d = T.hcp_slip_systems(["basal"])
+    np.testing.assert_allclose(n, [[0, 0, 1]] * 3, atol=1e-15)
+    np.testing.assert_allclose(d, [[1, 0, 0], [-0.5, r3b2, 0], [-0.5, -r3b2, 0]], atol=1e-15)
+    n, d = T.hcp_slip_systems(["prismatic"])
+    np.testing.assert_allclose(n, [[r3b2, 0.5, 0], [0, -1, 0], [-r3b2, 0.5, 0]], atol=1e-15)
+    np.testing.assert_allclose(d, [[-0.5, r3b2, 0], [1, 0, 0], [-0.5, -r3b2, 0]], atol=1e-15)
+    a_over_c = 2 / 3
+    rows = [[1, 1 / r3, a_over_c], [0, -2 / r3, a_over_c], [-1, 1 / r3, a_over_c],
+            [-1, -1 / r3, a_over_c], [0, 2 / r3, a_over_c], [1, -1 / r3, a_over_c]]
+    n, d = T.hcp_slip_systems(["pyramidal_a"], c_over_a=1.5)
+    np.testing.assert_allclose(n, np.array(rows) * 0.75, atol=1e-15)
+    np.testing.assert_allclose(d, [[-0.5, r3b2, 0], [1, 0, 0], [-0.5, -r3b2, 0],
+                                   [-0.5, r3b2, 0], [1, 0, 0], [0.5, r3b2, 0]], atol=1e-15)
+    n, d = T.hcp_slip_systems(["pyramidal_ca"], c_over_a=1.5)
+    np.testing.assert_allclose(n, np.repeat(np.array(rows), 2, axis=0) * 0.75, atol=1e-15)
+    ca = 1.5
+    dirs = np.array([[-1.5, -3 * r3b2, 3 * ca], [-3, 0, 3 * ca], [1.5, 3 * r3b2, 3 * ca],
+                     [-1.5, 3 * r3b2, 3 * ca], [3, 0, 3 * ca], [1.5, -3 * r3b2, 3 * ca],
+                     [3, 0, 3 * ca], [1.5, 3 * r3b2, 3 * ca], [-1.5, -3 * r3b2, 3 * ca],
+                     [1.5, -3 * r3b2, 3 * ca], [-3, 0, 3 * ca], [-1.5, 3 * r3b2, 3 * ca]])
+    np.testing.assert_allclose(d, dirs * 2 / (3 * np.sqrt(13)), atol=1e-15)
+    # maximum Schmid factors along [111] (tests/test_slip_system.py:148-190)
+    l111 = [1, 1, 1]
+    assert np.isclose(T.max_schmid_factor(*T.octahedral_slip_systems(), l111), 2 / (3 * np.sqrt(6)))
+    assert np.isclose(T.max_schmid_factor(*T.hcp_slip_systems(["basal"]), l111), (1 + r3) / 6)
+    assert np.isclose(T.max_schmid_factor(*T.hcp_slip_systems(["prismatic"]), l111), 1 / 3)
+    assert np.isclose(T.max_schmid_factor(*T.hcp_slip_systems(["pyramidal_a"], 1.5), l111),
+                      (1 + r3) / 6)
+    assert np.isclose(T.max_schmid_factor(*T.hcp_slip_systems(["pyramidal_ca"], 1.5), l111),
+                      (7 + 5 * r3) / (12 * np.sqrt(13)))
