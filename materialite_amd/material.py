@@ -14,6 +14,8 @@ million objects; ``extract`` returns the same tensor types as the reference.
 
 from copy import deepcopy
 
+import warnings
+
 import numpy as np
 import pandas as pd
 from scipy import spatial
@@ -212,6 +214,26 @@ class Material:
         return self.create_regional_fields(
             region_label, {region_label: unique_regions, orientation_label: orientations})
 
+    def export_to_evpfft(self, orientation_label="orientation", grain_label="grain",
+                         phase_label="phase", output="fields.txt", euler_angles_to_degrees=True):
+        """EVP-FFT microstructure file (material.py:1035-1091): one line per point,
+        "e1 e2 e3 x y z grain phase" with Bunge angles (%.5f), 1-based voxel ids,
+        sorted z slowest, x fastest, space separated."""
+        dims = self.dimensions
+        if not np.array_equal([power_of_two_below(d) for d in dims], dims):
+            warnings.warn("Outputting to EVP-FFT with dimensions that are not powers of 2. Use "
+                          "Material.crop_by_range() or Material.crop_by_id_range() if the "
+                          "dimensions must be powers of 2.")
+        o = self.extract(orientation_label)
+        e = np.atleast_2d(o.euler_angles_in_degrees if euler_angles_to_degrees else o.euler_angles)
+        table = pd.DataFrame({
+            "euler_angle_1": e[:, 0], "euler_angle_2": e[:, 1], "euler_angle_3": e[:, 2],
+            "x": self.extract("x_id") + 1, "y": self.extract("y_id") + 1,
+            "z": self.extract("z_id") + 1,
+            "grain": self.extract(grain_label), "phase": self.extract(phase_label),
+        }).sort_values(by=["z", "y", "x"])
+        table.to_csv(output, header=False, sep=" ", index=False, float_format="%.5f")
+
     def insert_feature(self, feature, fields):
         """Set field values at points inside a feature (material.py:421-470)."""
         pts = self.fields[["x", "y", "z"]].to_numpy()
@@ -264,3 +286,32 @@ class Box:
 
 __all__ = ["Material", "Sphere", "Box", "Vector", "Scalar", "Order2SymmetricTensor",
            "Order4SymmetricTensor", "Orientation"]
+
+
+# ------------------------------------------------------------------ EVP-FFT text files
+def power_of_two_below(n):
+    """Largest power of two <= n (material.py, util)."""
+    return 1 << (int(n).bit_length() - 1)
+
+
+def import_evpfft(file, spacing=(1, 1, 1), origin=(0, 0, 0), grain_label="grain",
+                  phase_label="phase", orientation_label="orientation",
+                  euler_angles_to_radians=True):
+    """Material from an EVP-FFT microstructure file (importers.py:153-200):
+    dimensions from the largest 1-based voxel ids, Bunge angles in degrees
+    (radians when euler_angles_to_radians is False)."""
+    table = pd.read_csv(file, header=None, delimiter=" ",
+                        names=["e1", "e2", "e3", "x_id", "y_id", "z_id", grain_label,
+                               phase_label]).sort_values(by=["x_id", "y_id", "z_id"])
+    dims = table[["x_id", "y_id", "z_id"]].max().to_numpy()
+    angles = table[["e1", "e2", "e3"]].to_numpy()
+    orientations = Orientation.from_euler_angles(angles, in_degrees=euler_angles_to_radians)
+    material = Material(dimensions=dims, spacing=spacing, origin=origin)
+    if len(table) != material.num_points:
+        raise ValueError("EVP-FFT file does not cover the full voxel grid")
+    return material.create_fields({
+        grain_label: table[grain_label].to_numpy(),
+        phase_label: table[phase_label].to_numpy(),
+        orientation_label: Orientation(np.reshape(orientations.rotation_matrix, (-1, 3, 3)), "p"),
+    })
+

@@ -497,11 +497,27 @@ class Orientation:
 
     @property
     def euler_angles(self):
+        """Bunge (z1, x2, z3) in radians (tensor.py:1919-1956, D. Eberly's
+        "Euler Angle Formulas"): the general case for -1 < R22 < 1, and
+        x2 = 0 or pi with z3 = 0 on the two degenerate sides."""
         r = self.rotation_matrix
-        x2 = np.arccos(np.clip(r[..., 2, 2], -1.0, 1.0))
-        z1 = np.arctan2(r[..., 2, 0], -r[..., 2, 1])
-        z3 = np.arctan2(r[..., 0, 2], r[..., 1, 2])
-        return np.stack([z1, x2, z3], axis=-1)
+        r22 = r[..., 2, 2]
+        general = np.logical_and(r22 < 1.0, r22 > -1.0)
+        with np.errstate(invalid="ignore"):
+            out = np.stack([np.arctan2(r[..., 2, 0], -r[..., 2, 1]), np.arccos(r22),
+                            np.arctan2(r[..., 0, 2], r[..., 1, 2])], axis=-1)
+        if np.all(general):
+            return np.squeeze(out)
+        zeros = np.zeros(r22.shape)
+        lower = np.stack([np.arctan2(r[..., 1, 0], r[..., 0, 0]), np.pi * np.ones(r22.shape),
+                          zeros], axis=-1)
+        upper = np.stack([np.arctan2(-r[..., 1, 0], r[..., 0, 0]), zeros, zeros], axis=-1)
+        out = np.where(general[..., None], out, np.where((r22 >= 1.0)[..., None], upper, lower))
+        return np.squeeze(out)
+
+    @property
+    def euler_angles_in_degrees(self):
+        return self.euler_angles * 180.0 / np.pi
 
     @property
     def rotation_matrix_mandel(self):
