@@ -103,6 +103,43 @@ struct Dft<T, 8> {
   }
 };
 
+// 16 = 4 x 4: DFT4 over n1 of x[4 n1 + n2], twiddle W16^(n2 k1), DFT4 over n2;
+// output k1 + 4 k2.  The W16 powers come from the L-point table (rt[m L/16]),
+// W16^4 = -i exactly.
+template <typename T>
+struct Dft<T, 16> {
+  __device__ __forceinline__ static void run(cplx<T>* v, bool inv, const cplx<T>* rt, int q) {
+    cplx<T> a[4][4];
+#pragma unroll
+    for (int n2 = 0; n2 < 4; ++n2) {
+      cplx<T> t[4] = {v[n2], v[4 + n2], v[8 + n2], v[12 + n2]};
+      Dft<T, 4>::run(t, inv);
+#pragma unroll
+      for (int k1 = 0; k1 < 4; ++k1) a[n2][k1] = t[k1];
+    }
+#pragma unroll
+    for (int n2 = 1; n2 < 4; ++n2)
+#pragma unroll
+      for (int k1 = 1; k1 < 4; ++k1) {
+        const int m = n2 * k1;
+        if (m == 4) {
+          a[n2][k1] = rot_i(a[n2][k1], inv);
+        } else {
+          cplx<T> w = rt[m * q];
+          if (inv) w.y = -w.y;
+          a[n2][k1] = cmul(a[n2][k1], w);
+        }
+      }
+#pragma unroll
+    for (int k1 = 0; k1 < 4; ++k1) {
+      cplx<T> t[4] = {a[0][k1], a[1][k1], a[2][k1], a[3][k1]};
+      Dft<T, 4>::run(t, inv);
+#pragma unroll
+      for (int k2 = 0; k2 < 4; ++k2) v[k1 + 4 * k2] = t[k2];
+    }
+  }
+};
+
 template <typename T>
 struct Dft<T, 3> {
   __device__ __forceinline__ static void run(cplx<T>* v, bool inv, const cplx<T>* rt, int q) {
@@ -214,6 +251,7 @@ __device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int
       const int jq = fast_div(j, mg);
       const int kk = j - jq * Ns;
       switch (R) {
+        case 16: butterfly<T, 16>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
         case 8: butterfly<T, 8>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
         case 4: butterfly<T, 4>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
         case 2: butterfly<T, 2>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
@@ -237,13 +275,13 @@ __device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int
   return a;
 }
 
-// Host: factor L into radices 8,4,2,3,5,7 (largest first). Throws if impossible.
+// Host: factor L into radices 16,8,4,2,3,5,7 (largest first). Throws if impossible.
 inline AxisPlan make_axis_plan(int L) {
   AxisPlan p{};
   p.L = L;
   p.npass = 0;
   int rem = L, ns = 1;
-  const int order[6] = {8, 4, 2, 3, 5, 7};
+  const int order[7] = {16, 8, 4, 2, 3, 5, 7};
   for (int r : order) {
     while (rem % r == 0 && rem > 1) {
       if (p.npass >= kMaxPasses) throw Error("FFT plan: too many passes");

@@ -26,6 +26,10 @@ namespace {
 
 // threads per workgroup of each pass
 constexpr int kZBT = 256, kYBT = 256, kXBT = 256;
+// elements of one component tile per thread in the x pass (nx * TX <= kXTileK * kXBT)
+constexpr int kXTileK = 8;
+// LDS bytes of the x pass's two single-component buffers (doubled for FP64)
+constexpr size_t kXPairBytes = 40 * 1024;
 // LDS bytes per FP32 workgroup for the two ping-pong line buffers (doubled for
 // FP64).  Small tiles keep many workgroups resident per CU (a CU has 160 KB),
 // which hides the HBM latency of the strided tile loads: measured at 128^3,
@@ -36,6 +40,13 @@ constexpr size_t kMaxLds = 160 * 1024;
 // LDS row stride of the z passes: 3 interleaved lines (paired components) per
 // grid line, rounded up to odd
 __host__ __device__ inline int zrow_stride(int P) { return (3 * P) | 1; }
+// LDS row stride of the x pass for TX interleaved lines: odd, so that the
+// butterfly stores of a wavefront spread over the LDS banks
+__host__ __device__ inline int odd_stride(int M) { return M | 1; }
+// LDS bytes of the x pass: twiddles and two single-component buffers
+inline size_t x_lds_bytes(int nx, int TX, size_t cb) {
+  return (size_t(nx) + 2 * size_t(nx) * size_t(odd_stride(TX))) * cb;
+}
 // complex values in each of the two z-pass buffers: the FFT rows, and in
 // z_inverse also the staged half spectra of the P lines (6 components)
 __host__ __device__ inline int zbuf_elems(int P, int nz, int nzc) {
@@ -175,8 +186,12 @@ __device__ __forceinline__ double rnorm3<double>(double a, double b, double c) {
   return 1.0 / sqrt(a * a + b * b + c * c);
 }
 
+// Tile = (y, TX consecutive kz columns) with all six components resident in
+// LDS: forward x-FFT of the 6 TX interleaved lines, Ghat on each point, inverse
+// x-FFT, one HBM read and one write of the tile.  Used where TX >= 2 fits the
+// budget (grids up to about 128 points along x).
 template <typename T>
-__global__ void __launch_bounds__(kXBT) k_x_ghat(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
+__global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
                                                   const T* tanz, int maskbits,
                                                   const MinresState* st, cplx<T>* spec) {
@@ -239,6 +254,103 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat(Grid g, const AxisPlan* plan_g,
     const int c = cxp / nx, x = cxp - c * nx;
     if (t < width) base[c * g.nspec + x * xstride + t] = out[x * (6 * TX) + c * TX + t];
   }
+}
+
+// Tile = (y, TX consecutive kz columns of 128 B where the grid allows), for
+// grids too long along x for six resident components at TX >= 2.  The
+// six components go through the LDS one at a time (forward x-FFT, stored back
+// in place), Ghat then runs on the tile's points straight from memory (the
+// tile was just written by this workgroup and is L2-resident), and the six
+// components are inverse-transformed one at a time.  While component c is
+// transformed, the registers already hold the loads of component c + 1.
+// Keeping one component in LDS (instead of all six) keeps tiles 128 B wide and
+// four workgroups per CU.
+template <typename T>
+__global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
+                                                  int ntiles, const T* tanx, const T* tany,
+                                                  const T* tanz, int maskbits,
+                                                  const MinresState* st, cplx<T>* spec) {
+  if (st != nullptr && st->done) return;
+  extern __shared__ unsigned char smem_raw[];
+  const AxisPlan* plan = plan_g;
+  const int nx = g.nx;
+  cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
+  const int S = odd_stride(TX);
+  cplx<T>* buf = tw + nx;
+  cplx<T>* buf2 = buf + nx * S;
+  const int b = xcd_logical_block(blockIdx.x, gridDim.x);
+  const int tile = b % ntiles;
+  const int y = b / ntiles;
+  const int kz0 = tile * TX;
+  const int width = min(TX, g.nzc - kz0);
+  const long long xstride = (long long)g.ny * g.nzc;
+  cplx<T>* base = spec + (long long)y * g.nzc + kz0;  // spec[c][0][y][kz0]
+  load_twiddles(tw, twx, nx);
+  // per-thread elements of one component tile: (x, t) with t fastest
+  constexpr int K = kXTileK;
+  const int E = nx * TX;
+  int goff[K], loff[K];
+  bool on[K];
+#pragma unroll
+  for (int k = 0; k < K; ++k) {
+    const int i = threadIdx.x + k * kXBT;
+    const int x = i / TX, t = i - x * TX;
+    goff[k] = x * int(xstride) + t;
+    loff[k] = x * S + t;
+    on[k] = i < E && t < width;
+  }
+  cplx<T> pre[K];
+  auto fetch = [&](int c) {
+    const cplx<T>* src = base + c * g.nspec;
+#pragma unroll
+    for (int k = 0; k < K; ++k) pre[k] = on[k] ? src[goff[k]] : cplx<T>{T(0), T(0)};
+  };
+  auto transform = [&](bool inv) {
+    fetch(0);
+    for (int c = 0; c < 6; ++c) {
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (threadIdx.x + k * kXBT < E) buf[loff[k]] = pre[k];
+      lds_barrier();
+      if (c < 5) fetch(c + 1);
+      const cplx<T>* res = block_fft<T, kXBT>(buf, buf2, TX, S, plan, tw, inv);
+      cplx<T>* dst = base + c * g.nspec;
+#pragma unroll
+      for (int k = 0; k < K; ++k)
+        if (on[k]) dst[goff[k]] = res[loff[k]];
+      lds_barrier();
+    }
+  };
+  transform(false);
+  __syncthreads();  // the tile's transformed components are visible to every wave
+  const int nyqx = (nx % 2 == 0) ? nx / 2 : -1;
+  const int nyqy = (g.ny % 2 == 0) ? g.ny / 2 : -1;
+  const int nyqz = (g.nz % 2 == 0) ? g.nz / 2 : -1;  // never stored, kept for clarity
+  const T ty = tany[y];
+  for (int i = threadIdx.x; i < nx * TX; i += kXBT) {
+    const int t = i % TX, kx = i / TX;
+    const int kz = kz0 + t;
+    if (t >= width) continue;
+    cplx<T>* pt = base + kx * xstride + t;
+    cplx<T> X[6];
+#pragma unroll
+    for (int c = 0; c < 6; ++c) X[c] = pt[c * g.nspec];
+    if (kx == nyqx || y == nyqy || kz == nyqz) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) X[c] = {T(0), T(0)};
+    } else if (kx == 0 && y == 0 && kz == 0) {
+#pragma unroll
+      for (int c = 0; c < 6; ++c) X[c] = (maskbits >> c) & 1 ? X[c] : cplx<T>{T(0), T(0)};
+    } else {
+      const T tx = tanx[kx], tz = tanz[kz];
+      const T inv = rnorm3<T>(tx, ty, tz);
+      apply_ghat<T>(X, tx * inv, ty * inv, tz * inv);
+    }
+#pragma unroll
+    for (int c = 0; c < 6; ++c) pt[c * g.nspec] = X[c];
+  }
+  __syncthreads();
+  transform(true);
 }
 
 // ----------------------------------------------------------------- z inverse
@@ -404,9 +516,25 @@ Spectral<T>::Spectral(int nx, int ny, int nz) {
   // y pass: ty_ kz columns per workgroup
   if ((size_t(ny) + 2 * size_t(ny)) * cb > kMaxLds) throw Error("ny too large for the y pass");
   ty_ = int(std::max<size_t>(1, std::min<size_t>({16, size_t(g_.nzc), ybudget / (2 * ny * cb)})));
-  // x pass: tx_ kz columns of all 6 components per workgroup
-  if ((size_t(nx) + 2 * 6 * size_t(nx)) * cb > kMaxLds) throw Error("nx too large for the x pass");
-  tx_ = int(std::max<size_t>(1, std::min<size_t>({8, size_t(g_.nzc), xbudget / (2 * 6 * nx * cb)})));
+  // x pass: tx_ kz columns of all 6 components resident per workgroup where
+  // at least two fit the budget; otherwise the components stream through LDS
+  // one at a time with tx_ the widest (up to 128 B rows) tile whose two
+  // single-component buffers fit
+  tx_ = int(std::min<size_t>({8, size_t(g_.nzc), xbudget / (2 * 6 * nx * cb)}));
+  x_resident_ = tx_ >= std::min(2, g_.nzc);
+  if (!x_resident_) {
+    tx_ = 0;
+    for (int t = int(128 / cb); t >= 1; t /= 2)
+      if (x_lds_bytes(nx, t, cb) <= kXPairBytes * scale &&
+          size_t(nx) * t <= size_t(kXTileK) * kXBT) {
+        tx_ = t;
+        break;
+      }
+    if (tx_ == 0 && x_lds_bytes(nx, 1, cb) <= kMaxLds && size_t(nx) <= size_t(kXTileK) * kXBT)
+      tx_ = 1;
+    if (tx_ == 0) throw Error("nx too large for the x pass");
+    tx_ = std::min(tx_, g_.nzc);
+  }
   MTL_HIP(hipMalloc(&spec_, 6 * g_.nspec * sizeof(cplx<T>)));
   twx_ = upload(twiddle_table<T>(nx));
   twy_ = upload(twiddle_table<T>(ny));
@@ -459,8 +587,9 @@ void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const V
   profiler().end(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), s);
 
   const int xtiles = ceil_div(g_.nzc, tx_);
-  const size_t xlds = (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<T>);
-  auto kx = k_x_ghat<T>;
+  const size_t xlds = x_resident_ ? (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<T>)
+                                  : x_lds_bytes(g_.nx, tx_, sizeof(cplx<T>));
+  auto kx = x_resident_ ? k_x_ghat_resident<T> : k_x_ghat_streamed<T>;
   allow_lds(kx, xlds);
   profiler().begin(s);
   hipLaunchKernelGGL(kx, dim3(g_.ny * xtiles), dim3(kXBT), xlds, s, g_, pxd_, twx_, tx_, xtiles,

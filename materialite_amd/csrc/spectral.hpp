@@ -51,6 +51,7 @@ class Spectral {
   int pz_lines_;  // z-lines per workgroup in the z passes
   int ty_;        // kz columns per workgroup in the y pass
   int tx_;        // kz columns per workgroup in the x pass
+  bool x_resident_;  // x pass keeps all six components of its tile in LDS
   cplx<T>* spec_ = nullptr;
   cplx<T>* twx_ = nullptr;
   cplx<T>* twy_ = nullptr;
