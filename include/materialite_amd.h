@@ -145,8 +145,8 @@ int mtl_solver_constitutive(mtl_solver* h, const double* strain_dev, const doubl
 
 /* ---- kernel timing (no counterpart in the reference, which has no profiler
  * hooks; used by bench.py for the roofline figures) ----
- * HIP events are recorded around each library kernel on its launch stream
- * while enabled.  kernel ids: 0 z-forward FFT pass, 1 y FFT pass, 2 x FFT pass
+ * While enabled, HIP start/stop events ride on the dispatch of each library
+ * kernel (on its launch stream).  kernel ids: 0 z-forward FFT pass, 1 y FFT pass, 2 x FFT pass
  * with Ghat, 3 z-inverse FFT pass, 4 MINRES vector updates, 5 MINRES scalar
  * steps, 6 EVP iteration, 7 EVP outputs, 8 elastic update, 9 Newton
  * reductions, 10-13 the four FFT passes (ids 0-3) when they project a
@@ -154,6 +154,10 @@ int mtl_solver_constitutive(mtl_solver* h, const double* strain_dev, const doubl
 int mtl_profile_enable(int enable);
 int mtl_profile_reset(void);
 int mtl_profile_read(int kernel, double* total_ms, long long* launches);
+/* Milliseconds of each timed launch of `kernel` since the last reset, in
+ * launch order: writes min(capacity, count) values and sets *count to the
+ * number recorded.  No reference counterpart (measurement only). */
+int mtl_profile_durations(int kernel, float* ms_out, long long capacity, long long* count);
 
 #ifdef __cplusplus
 }

@@ -104,6 +104,8 @@ EXPORTS = {
     "mtl_profile_reset": (ctypes.c_int, []),
     "mtl_profile_read": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_longlong)]),
+    "mtl_profile_durations": (ctypes.c_int, [ctypes.c_int, ctypes.c_void_p, ctypes.c_longlong,
+                                             ctypes.POINTER(ctypes.c_longlong)]),
 }
 
 KERNELS = ["z_forward", "y_pass", "x_ghat", "z_inverse", "minres_vector", "minres_scalar",
@@ -129,6 +131,21 @@ def profile_read():
         check(lib.mtl_profile_read(i, ctypes.byref(ms), ctypes.byref(n)))
         out[name] = (ms.value, n.value)
     return out
+
+def profile_durations(kernel):
+    """Milliseconds of each timed launch of `kernel` (a name of KERNELS) since
+    the last reset, as a float32 array in launch order."""
+    import numpy as np
+
+    lib = load_library()
+    i = KERNELS.index(kernel)
+    n = ctypes.c_longlong()
+    check(lib.mtl_profile_durations(i, None, 0, ctypes.byref(n)))
+    out = np.zeros(n.value, dtype=np.float32)
+    if n.value:
+        check(lib.mtl_profile_durations(i, out.ctypes.data, n.value, ctypes.byref(n)))
+    return out
+
 
 _LIB = None
 

@@ -644,11 +644,8 @@ class ElasticModel final : public Model<K> {
   void launch(const double* rot9, const double* eps, double* sig, K* tangent, int ws,
               hipStream_t s) {
     if (this->n_ == 0) return;
-    profiler().begin(s);
-    hipLaunchKernelGGL(k_elastic<K>, dim3(ceil_div(this->n_, kCBT)), dim3(kCBT), 0, s, this->n_,
-                       this->idx_, this->npts_, rot9, C_, eps, sig, tangent, ws);
+    profiler().launch(kKElastic, k_elastic<K>, dim3(ceil_div(this->n_, kCBT)), dim3(kCBT), 0, s, this->n_, this->idx_, this->npts_, rot9, C_, eps, sig, tangent, ws);
     MTL_LAUNCH_CHECK();
-    profiler().end(kKElastic, s);
   }
   Stiff21 C_;
 };
@@ -756,20 +753,14 @@ class EvpModel final : public Model<K> {
     while (true) {
       for (int k = 0; k < chunk && it < kEvpMaxIterations; ++k) {
         ++it;
-        profiler().begin(s);
-        if (q_.nint >= 0)
-          hipLaunchKernelGGL((k_evp_iter<NS, true>), dim3(blocks), dim3(kCBT), 0, s, it, n, qd_, B_,
-                             ctrl_);
-        else
-          hipLaunchKernelGGL((k_evp_iter<NS, false>), dim3(blocks), dim3(kCBT), 0, s, it, n, qd_, B_,
-                             ctrl_);
+        profiler().launch(kKEvpIter, q_.nint >= 0 ? k_evp_iter<NS, true> : k_evp_iter<NS, false>,
+                          dim3(blocks), dim3(kCBT), 0, s, it, n, qd_, B_, ctrl_);
         MTL_LAUNCH_CHECK();
-        profiler().end(kKEvpIter, s);
         hipLaunchKernelGGL(k_evp_control, dim3(1), dim3(64), 0, s, it, n, ctrl_);
         MTL_LAUNCH_CHECK();
       }
       MTL_HIP(hipMemcpyAsync(hctrl_, ctrl_, sizeof(EvpControl), hipMemcpyDeviceToHost, s));
-      MTL_HIP(hipStreamSynchronize(s));
+      stream_sync(s);
       if (hctrl_->done || it >= kEvpMaxIterations) break;
       chunk = 4;
     }
@@ -781,15 +772,11 @@ class EvpModel final : public Model<K> {
                    hctrl_->fail);
     }
     if (hctrl_->fail || !hctrl_->done) return 0;
-    profiler().begin(s);
-    if (q_.nint >= 0)
-      hipLaunchKernelGGL((k_evp_outputs<NS, K, true>), dim3(blocks), dim3(kCBT), 0, s, n,
-                         this->idx_, this->npts_, rot9, qd_, B_, ctrl_, sig, tangent);
-    else
-      hipLaunchKernelGGL((k_evp_outputs<NS, K, false>), dim3(blocks), dim3(kCBT), 0, s, n,
-                         this->idx_, this->npts_, rot9, qd_, B_, ctrl_, sig, tangent);
+    profiler().launch(kKEvpFinish,
+                      q_.nint >= 0 ? k_evp_outputs<NS, K, true> : k_evp_outputs<NS, K, false>,
+                      dim3(blocks), dim3(kCBT), 0, s, n, this->idx_, this->npts_, rot9, qd_, B_,
+                      ctrl_, sig, tangent);
     MTL_LAUNCH_CHECK();
-    profiler().end(kKEvpFinish, s);
     // adopt the call's outputs as the trial state (elastic_viscoplastic.py:175-182)
     std::swap(B_.ep_pend, ep_tr_);
     std::swap(B_.g_pend, g_tr_);
@@ -816,7 +803,7 @@ class EvpModel final : public Model<K> {
       soa.resize(size_t(nc) * n);
       if (n) MTL_HIP(hipMemcpyAsync(soa.data(), src, soa.size() * sizeof(double),
                                     hipMemcpyDeviceToHost, s));
-      MTL_HIP(hipStreamSynchronize(s));
+      stream_sync(s);
       out.resize(size_t(nc) * n);
       for (int i = 0; i < n; ++i)
         for (int c = 0; c < nc; ++c) out[size_t(i) * nc + c] = soa[size_t(c) * n + i];
@@ -833,7 +820,7 @@ class EvpModel final : public Model<K> {
         out.resize(size_t(6) * n);
         if (n) MTL_HIP(hipMemcpyAsync(out.data(), tmp, out.size() * sizeof(double),
                                       hipMemcpyDeviceToHost, s));
-        MTL_HIP(hipStreamSynchronize(s));
+        stream_sync(s);
         (void)hipFree(tmp);
         ncomp = 6;
         return true;
@@ -1117,13 +1104,10 @@ class J2Model final : public Model<K> {
     const int blocks = ceil_div(n, kCBT);
     hipLaunchKernelGGL(k_j2_ctrl_reset, dim3(1), dim3(64), 0, s, ctl_);
     MTL_LAUNCH_CHECK();
-    profiler().begin(s);
-    hipLaunchKernelGGL(k_j2_scan, dim3(blocks), dim3(kCBT), 0, s, n, this->idx_, this->npts_, qd_,
-                       eps, B_, ctl_);
+    profiler().launch(kKElastic, k_j2_scan, dim3(blocks), dim3(kCBT), 0, s, n, this->idx_, this->npts_, qd_, eps, B_, ctl_);
     MTL_LAUNCH_CHECK();
-    profiler().end(kKElastic, s);
     MTL_HIP(hipMemcpyAsync(hctl_, ctl_, sizeof(J2Control), hipMemcpyDeviceToHost, s));
-    MTL_HIP(hipStreamSynchronize(s));
+    stream_sync(s);
     int iters = 1;
     if (hctl_->plastic) {
       iters = 0;
@@ -1134,11 +1118,8 @@ class J2Model final : public Model<K> {
         }
       if (iters == 0) return 0;  // "Too many iterations in material loop" (:128-130)
     }
-    profiler().begin(s);
-    hipLaunchKernelGGL(k_j2_finish<K>, dim3(blocks), dim3(kCBT), 0, s, n, this->idx_, this->npts_,
-                       qd_, eps, iters, hctl_->plastic, B_, sig, tangent);
+    profiler().launch(kKElastic, k_j2_finish<K>, dim3(blocks), dim3(kCBT), 0, s, n, this->idx_, this->npts_, qd_, eps, iters, hctl_->plastic, B_, sig, tangent);
     MTL_LAUNCH_CHECK();
-    profiler().end(kKElastic, s);
     if (hctl_->plastic) {
       // the call's state becomes the trial state (isotropic_elastic_plastic.py:131-137)
       std::swap(B_.ep_pend, ep_tr_);
@@ -1164,7 +1145,7 @@ class J2Model final : public Model<K> {
       std::vector<double> soa(size_t(nc) * n);
       if (n) MTL_HIP(hipMemcpyAsync(soa.data(), src, soa.size() * sizeof(double),
                                     hipMemcpyDeviceToHost, s));
-      MTL_HIP(hipStreamSynchronize(s));
+      stream_sync(s);
       out.resize(size_t(nc) * n);
       for (int i = 0; i < n; ++i)
         for (int c = 0; c < nc; ++c) out[size_t(i) * nc + c] = soa[size_t(c) * n + i];

@@ -241,30 +241,19 @@ int Minres<T>::solve(const double* b, const T* tangent, const Vec6& mask, double
   while (true) {
     for (int it = 0; it < chunk; ++it) {
       const int np = sp_.matvec_minres(r2, tangent, r1, mask, state_, partials_, s);
-      profiler().begin(s);
-      hipLaunchKernelGGL(k_alpha, dim3(1), dim3(kVBT), 0, s, state_, partials_, np);
+      profiler().launch(kKMinresScalar, k_alpha, dim3(1), dim3(kVBT), 0, s, state_, partials_, np);
       MTL_LAUNCH_CHECK();
-      profiler().end(kKMinresScalar, s);
-      profiler().begin(s);
-      hipLaunchKernelGGL(k_lanczos<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, r2, partials_);
+      profiler().launch(kKMinresVector, k_lanczos<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, r2, partials_);
       MTL_LAUNCH_CHECK();
-      profiler().end(kKMinresVector, s);
-      profiler().begin(s);
-      hipLaunchKernelGGL(k_givens, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb);
+      profiler().launch(kKMinresScalar, k_givens, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb);
       MTL_LAUNCH_CHECK();
-      profiler().end(kKMinresScalar, s);
       // Lanczos roles rotate: r1 <- r2 (old), r2 <- y (stored in old r1)
       std::swap(r1, r2);
       // v_k = r2_old / beta_k now lives in r1
-      profiler().begin(s);
-      hipLaunchKernelGGL(k_update<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, w1, w2, wf, x_,
-                         partials_);
+      profiler().launch(kKMinresVector, k_update<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, w1, w2, wf, x_, partials_);
       MTL_LAUNCH_CHECK();
-      profiler().end(kKMinresVector, s);
-      profiler().begin(s);
-      hipLaunchKernelGGL(k_tests, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb);
+      profiler().launch(kKMinresScalar, k_tests, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb);
       MTL_LAUNCH_CHECK();
-      profiler().end(kKMinresScalar, s);
       // w roles rotate: w1 <- w2, w2 <- new w, free slot <- old w1
       T* old_w1 = w1;
       w1 = w2;
@@ -273,7 +262,7 @@ int Minres<T>::solve(const double* b, const T* tangent, const Vec6& mask, double
       ++launched;
     }
     MTL_HIP(hipMemcpyAsync(host_state_, state_, sizeof(MinresState), hipMemcpyDeviceToHost, s));
-    MTL_HIP(hipStreamSynchronize(s));
+    stream_sync(s);
     if (std::getenv("MTL_DEBUG") != nullptr) {
       const MinresState& z = *host_state_;
       std::fprintf(stderr, "[mtl]     minres itn %d beta %.4e alfa %.4e phibar %.4e test1 %.3e test2 %.3e\n",

@@ -269,7 +269,7 @@ class Solver final : public SolverBase {
     MTL_HIP(hipMemsetAsync(D_, 0, 21 * n_ * sizeof(K), s_));
     for (auto& m : models_) m->init_tangent(rot_, D_, s_);
     MTL_HIP(hipMemcpyAsync(D_old_, D_, 21 * n_ * sizeof(K), hipMemcpyDeviceToDevice, s_));
-    MTL_HIP(hipStreamSynchronize(s_));
+    stream_sync(s_);
   }
 
   // Multiphase dispatch (multiphase.py:38-51): fails as soon as a model fails
@@ -294,7 +294,7 @@ class Solver final : public SolverBase {
     MTL_LAUNCH_CHECK();
     MTL_HIP(hipMemcpyAsync(hpart_, partials_, nb * kNormWidth * sizeof(double),
                            hipMemcpyDeviceToHost, s_));
-    MTL_HIP(hipStreamSynchronize(s_));
+    stream_sync(s_);
     Norms r{};
     for (int i = 0; i < nb; ++i) {
       const double* q = hpart_ + i * kNormWidth;
@@ -323,7 +323,7 @@ class Solver final : public SolverBase {
     hipLaunchKernelGGL(k_tangent_sum<K>, dim3(nb), dim3(kNBT), 0, s_, n_, D_, partials_);
     MTL_LAUNCH_CHECK();
     MTL_HIP(hipMemcpyAsync(hpart_, partials_, nb * 21 * sizeof(double), hipMemcpyDeviceToHost, s_));
-    MTL_HIP(hipStreamSynchronize(s_));
+    stream_sync(s_);
     double sum[21] = {};
     for (int i = 0; i < nb; ++i)
       for (int k = 0; k < 21; ++k) sum[k] += hpart_[i * 21 + k];
@@ -410,7 +410,7 @@ class Solver final : public SolverBase {
         hipLaunchKernelGGL(k_sum6<K>, dim3(nb), dim3(kNBT), 0, s_, n_, minres_->x(), partials_);
         MTL_LAUNCH_CHECK();
         MTL_HIP(hipMemcpyAsync(hpart_, partials_, nb * 6 * sizeof(double), hipMemcpyDeviceToHost, s_));
-        MTL_HIP(hipStreamSynchronize(s_));
+        stream_sync(s_);
         for (int c = 0; c < 6; ++c) {
           double sum = 0.0;
           for (int i = 0; i < nb; ++i) sum += hpart_[i * 6 + c];
@@ -463,12 +463,12 @@ class Solver final : public SolverBase {
     copy6(eps_old_, eps_);
     MTL_HIP(hipMemcpyAsync(D_old_, D_, 21 * n_ * sizeof(K), hipMemcpyDeviceToDevice, s_));
     scale6(fluct_, scale);
-    MTL_HIP(hipStreamSynchronize(s_));
+    stream_sync(s_);
   }
 
   void rollback() override {
     scale6(fluct_, 0.5);
-    MTL_HIP(hipStreamSynchronize(s_));
+    stream_sync(s_);
   }
 
   void get_field(int which, double* out, int* ncomp) override {
@@ -513,7 +513,7 @@ class Solver final : public SolverBase {
     Vec6 mask, zero{};
     std::memcpy(mask.v, m, sizeof(mask.v));
     spk_->project(in, out, zero, 1.0, mask, s_);
-    MTL_HIP(hipStreamSynchronize(s_));
+    stream_sync(s_);
   }
 
   int minres(const double* b, double* x, const double m[6], double rtol) override {
@@ -524,13 +524,13 @@ class Solver final : public SolverBase {
     hipLaunchKernelGGL((k_convert<K, double>), dim3(grid_for(6 * n_)), dim3(kNBT), 0, s_, 6 * n_,
                        minres_->x(), x);
     MTL_LAUNCH_CHECK();
-    MTL_HIP(hipStreamSynchronize(s_));
+    stream_sync(s_);
     return itn;
   }
 
   int constitutive(const double* eps, const double* guess, double dt, double* sig) override {
     const int it = constitutive_all(eps, guess, dt, sig);
-    MTL_HIP(hipStreamSynchronize(s_));
+    stream_sync(s_);
     return it;
   }
 
@@ -548,7 +548,7 @@ class Solver final : public SolverBase {
   void soa_to_host(const double* d, int nc, double* out) {
     std::vector<double> soa(size_t(nc) * n_);
     MTL_HIP(hipMemcpyAsync(soa.data(), d, soa.size() * sizeof(double), hipMemcpyDeviceToHost, s_));
-    MTL_HIP(hipStreamSynchronize(s_));
+    stream_sync(s_);
     for (long long p = 0; p < n_; ++p)
       for (int c = 0; c < nc; ++c) out[p * nc + c] = soa[size_t(c) * n_ + p];
   }
@@ -757,6 +757,18 @@ int mtl_profile_read(int kernel, double* total_ms, long long* launches) {
     mtl::profiler().flush();
     *total_ms = mtl::profiler().total_ms[kernel];
     *launches = mtl::profiler().launches[kernel];
+  });
+}
+
+int mtl_profile_durations(int kernel, float* ms_out, long long capacity, long long* count) {
+  if (!count || (capacity > 0 && !ms_out)) return bad("null argument");
+  if (kernel < 0 || kernel >= mtl::kNumKernelIds) return bad("unknown kernel id");
+  return guarded([&] {
+    mtl::profiler().flush();
+    const auto& d = mtl::profiler().durations[kernel];
+    *count = (long long)d.size();
+    const long long m = std::min<long long>(capacity, (long long)d.size());
+    for (long long i = 0; i < m; ++i) ms_out[i] = d[size_t(i)];
   });
 }
 

@@ -569,47 +569,32 @@ void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const V
   const size_t zlds = (size_t(g_.nz) + 2 * size_t(zbuf_elems(P, g_.nz, g_.nzc))) * sizeof(cplx<T>);
   auto kzf = k_z_forward<T, MODE, TIn>;
   allow_lds(kzf, zlds);
-  profiler().begin(s);
-  hipLaunchKernelGGL(kzf, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, in, tangent, a,
-                     sgn, st, spec_);
+  profiler().launch(MODE == kModeMatvec ? kKZForward : residual_id(kKZForward), kzf, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, in, tangent, a, sgn, st, spec_);
   MTL_LAUNCH_CHECK();
-  profiler().end(MODE == kModeMatvec ? kKZForward : residual_id(kKZForward), s);
 
   const MinresState* stp = MODE == kModeMatvec ? st : nullptr;
   const int ytiles = ceil_div(g_.nzc, ty_);
   const size_t ylds = (size_t(g_.ny) + 2 * size_t(ty_) * g_.ny) * sizeof(cplx<T>);
   auto ky = k_y_pass<T>;
   allow_lds(ky, ylds);
-  profiler().begin(s);
-  hipLaunchKernelGGL(ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles,
-                     false, stp, spec_);
+  profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, false, stp, spec_);
   MTL_LAUNCH_CHECK();
-  profiler().end(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), s);
 
   const int xtiles = ceil_div(g_.nzc, tx_);
   const size_t xlds = x_resident_ ? (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<T>)
                                   : x_lds_bytes(g_.nx, tx_, sizeof(cplx<T>));
   auto kx = x_resident_ ? k_x_ghat_resident<T> : k_x_ghat_streamed<T>;
   allow_lds(kx, xlds);
-  profiler().begin(s);
-  hipLaunchKernelGGL(kx, dim3(g_.ny * xtiles), dim3(kXBT), xlds, s, g_, pxd_, twx_, tx_, xtiles,
-                     tanx_, tany_, tanz_, mask_bits(mask), stp, spec_);
+  profiler().launch(MODE == kModeMatvec ? kKXGhat : residual_id(kKXGhat), kx, dim3(g_.ny * xtiles), dim3(kXBT), xlds, s, g_, pxd_, twx_, tx_, xtiles, tanx_, tany_, tanz_, mask_bits(mask), stp, spec_);
   MTL_LAUNCH_CHECK();
-  profiler().end(MODE == kModeMatvec ? kKXGhat : residual_id(kKXGhat), s);
 
-  profiler().begin(s);
-  hipLaunchKernelGGL(ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles,
-                     true, stp, spec_);
+  profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, true, stp, spec_);
   MTL_LAUNCH_CHECK();
-  profiler().end(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), s);
 
   auto kzi = k_z_inverse<T, MODE, TOut>;
   allow_lds(kzi, zlds);
-  profiler().begin(s);
-  hipLaunchKernelGGL(kzi, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, spec_, out, r1,
-                     reinterpret_cast<const T*>(in), st, partials);
+  profiler().launch(MODE == kModeMatvec ? kKZInverse : residual_id(kKZInverse), kzi, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, spec_, out, r1, reinterpret_cast<const T*>(in), st, partials);
   MTL_LAUNCH_CHECK();
-  profiler().end(MODE == kModeMatvec ? kKZInverse : residual_id(kKZInverse), s);
 }
 
 template <typename T>
