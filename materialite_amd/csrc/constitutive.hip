@@ -22,6 +22,13 @@
 // d = dt sum_s |rate_s| of the previous iteration, so one double per point
 // (d) replaces NS read and NS written values per iteration; recomputing g_s
 // from it repeats the same operations and is bitwise identical.
+//
+// Slip resistances are one value per point, not per system: the model takes a
+// scalar initial slip resistance (elastic_viscoplastic.py:66-69 adds it to
+// every system), and each hardening law (hardening_laws.py:18-90) applies the
+// same operations to every system of a point (the increment depends on the
+// point's total slip only, Armstrong-Frederick's also on g_old_s itself), so a
+// point's NS values stay bitwise equal for the whole run.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -125,6 +132,33 @@ __device__ __forceinline__ void store_tangent(K* tangent, long long npts, long l
                                               const double Ds[21]) {
 #pragma unroll
   for (int k = 0; k < 21; ++k) tangent[k * npts + p] = K(Ds[k]);
+}
+
+// tangent <- Rm^T Dc Rm as tangent_to_specimen + store_tangent (the same sums in
+// the same order), one column of Dc Rm at a time, each packed entry stored as
+// soon as it is formed: 6 intermediate values live instead of 36 + 21
+template <typename K>
+__device__ __forceinline__ void store_tangent_to_specimen(const double Rm[36], const double Dc[21],
+                                                          K* tangent, long long npts,
+                                                          long long p) {
+#pragma unroll
+  for (int b = 0; b < 6; ++b) {
+    double Tb[6];  // column b of Dc Rm
+#pragma unroll
+    for (int i = 0; i < 6; ++i) {
+      double acc = 0.0;
+#pragma unroll
+      for (int j = 0; j < 6; ++j) acc += Dc[sym6(i, j)] * Rm[j * 6 + b];
+      Tb[i] = acc;
+    }
+#pragma unroll
+    for (int a = 0; a <= b; ++a) {
+      double acc = 0.0;
+#pragma unroll
+      for (int i = 0; i < 6; ++i) acc += Rm[i * 6 + a] * Tb[i];
+      tangent[sym6(a, b) * npts + p] = K(acc);
+    }
+  }
 }
 
 struct Stiff21 {
@@ -272,10 +306,11 @@ __device__ __forceinline__ void spd_inverse6(const double L[21], double Sinv[21]
 }
 
 // One iteration of the per-point implicit update (elastic_viscoplastic.py:190-253),
-// accumulated one slip system at a time.  The per-system values live in LDS
-// columns (system s at col[s * kCBT]) rather than registers, so the loop over
-// systems need not be unrolled and the kernel keeps enough waves resident to
-// overlap its HBM traffic with the FP64 arithmetic.  On return S holds the
+// accumulated one slip system at a time (gs: the point's slip resistance).
+// The per-system slip increments (when asked for) go to LDS columns (system s
+// at dgam[s * kCBT]) rather than registers, so the loop over systems need not
+// be unrolled and the kernel keeps enough waves resident to overlap its HBM
+// traffic with the FP64 arithmetic.  On return S holds the
 // Cholesky factor of C^-1 + dt M, sig_new the updated stress, ep the plastic
 // strain, d the slip increment dt sum_s |rate_s|, and dgam (when non-null) the
 // slip increments rate_s dt.  The total and committed plastic strains are read
@@ -283,7 +318,7 @@ __device__ __forceinline__ void spd_inverse6(const double L[21], double Sinv[21]
 // them out of the registers live during it.
 template <int NS, bool IP>
 __device__ __forceinline__ void evp_iterate(const EvpParams<NS>& q, const double sig[6],
-                                            const double* g, const double* eps_p,
+                                            const double gs, const double* eps_p,
                                             const double* ep_old_p, long long ln,
                                             double sig_new[6], double ep[6], double S[21],
                                             double* dgam, double& diff, double& d) {
@@ -298,7 +333,6 @@ __device__ __forceinline__ void evp_iterate(const EvpParams<NS>& q, const double
     double tau = 0.0;
 #pragma unroll
     for (int m = 0; m < 6; ++m) tau += q.P[s][m] * sig[m];
-    const double gs = g[s * kCBT];
     const double pref = q.gamma0 * powi_abs<IP>(tau / gs, q.nint, q.n - 1.0) / gs;
     const double rate = pref * tau;
     const double w = q.n * pref;
@@ -336,40 +370,26 @@ __device__ __forceinline__ void evp_iterate(const EvpParams<NS>& q, const double
   d = q.dt * dsum;
 }
 
-// Slip resistances after a slip increment d from the committed state
-// (hardening_laws.py:18-90).
-// g_old and g_new are strided (system s at [s * stride]).
+// Slip resistance (every system of the point) after a slip increment d from
+// the committed state (hardening_laws.py:18-90).
 template <int NS>
-__device__ __forceinline__ void evp_harden(const EvpParams<NS>& q, const double* g_old,
-                                           long long old_stride, double acc_old, double d,
-                                           double* g_new, long long new_stride) {
-#define G_OLD(s) g_old[(s) * old_stride]
-#define G_NEW(s) g_new[(s) * new_stride]
-  if (q.hard == kLinear) {
-#pragma unroll
-    for (int s = 0; s < NS; ++s) G_NEW(s) = G_OLD(s) + q.hp[0] * d;
-  } else if (q.hard == kVoce) {
+__device__ __forceinline__ double evp_harden(const EvpParams<NS>& q, double g_old, double acc_old,
+                                             double d) {
+  if (q.hard == kLinear) return g_old + q.hp[0] * d;
+  if (q.hard == kVoce) {
     const double theta0 = q.hp[0], theta1 = q.hp[1], tau1 = q.hp[2];
     const double ratio = fabs(theta0 / tau1);
     const double e0 = exp(-ratio * acc_old);
     const double ed = exp(-ratio * d);
     const double inc = theta1 * d - (ratio * tau1 - theta1) / ratio * e0 * (ed - 1.0) -
                        theta1 / ratio * e0 * (ed * (1.0 + ratio * (acc_old + d)) - 1.0 - ratio * acc_old);
-#pragma unroll
-    for (int s = 0; s < NS; ++s) G_NEW(s) = G_OLD(s) + inc;
-  } else if (q.hard == kArmstrongFrederick) {
-    const double h = q.hp[0], r = q.hp[1];
-#pragma unroll
-    for (int s = 0; s < NS; ++s) {
-      const double go = G_OLD(s);
-      G_NEW(s) = go + (h - r * go) * d / (1.0 + r * d);
-    }
-  } else {
-#pragma unroll
-    for (int s = 0; s < NS; ++s) G_NEW(s) = G_OLD(s);
+    return g_old + inc;
   }
-#undef G_OLD
-#undef G_NEW
+  if (q.hard == kArmstrongFrederick) {
+    const double h = q.hp[0], r = q.hp[1];
+    return g_old + (h - r * g_old) * d / (1.0 + r * d);
+  }
+  return g_old;
 }
 
 template <int NS>
@@ -383,11 +403,11 @@ struct EvpBuffers {
   double* gnorm;     // [n]
   double* sig[2];    // [6][n] ping-pong
   double* dsl[2];    // [n] ping-pong slip increment d of the iteration
-  const double* g_trial;  // [NS][n] slip resistances of the first iteration
+  const double* g_trial;  // [n] slip resistance of the first iteration
   unsigned char* conv;
   int* last;         // [n] last iteration each point ran
   double* ep_old;    // [6][n]
-  double* g_old;     // [NS][n]
+  double* g_old;     // [n] (every system of a point has the same value)
   double* gam_old;   // [NS][n]
   double* acc_old;   // [n]
   double* ep_pend;   // outputs of the call, adopted as trial state on success
@@ -428,21 +448,16 @@ __global__ void k_evp_ctrl_reset(EvpControl* c) {
 }
 
 // State entering iteration it (1-based): the stress written by iteration it-1
-// (the initial guess for it = 1) and the slip resistances that iteration
-// produced (the trial values for it = 1), the latter into the LDS column g.
+// (the initial guess for it = 1) and the slip resistance that iteration
+// produced (the trial value for it = 1).
 template <int NS>
 __device__ __forceinline__ void load_point(const EvpParams<NS>& q, const EvpBuffers& B, int n,
-                                           int i, int it, double sig[6], double* g) {
+                                           int i, int it, double sig[6], double& g) {
   const long long ln = n;
   const int buf = (it - 1) & 1;
 #pragma unroll
   for (int c = 0; c < 6; ++c) sig[c] = B.sig[buf][c * ln + i];
-  if (it == 1) {
-#pragma unroll
-    for (int s = 0; s < NS; ++s) g[s * kCBT] = B.g_trial[s * ln + i];
-  } else {
-    evp_harden<NS>(q, B.g_old + i, ln, B.acc_old[i], B.dsl[buf][i], g, kCBT);
-  }
+  g = it == 1 ? B.g_trial[i] : evp_harden<NS>(q, B.g_old[i], B.acc_old[i], B.dsl[buf][i]);
 }
 
 // Iteration `it` (1-based) of every point that still iterates: all points
@@ -455,7 +470,6 @@ __global__ void __launch_bounds__(kCBT) k_evp_iter(int it, int n, const EvpParam
   if (ctrl->done) return;
   const int masked = ctrl->masked;
   __shared__ EvpParams<NS> q;
-  __shared__ double gsh[NS * kCBT];
   __shared__ int cnt;
   load_params(&q, qg);
   if (threadIdx.x == 0) cnt = 0;
@@ -464,8 +478,7 @@ __global__ void __launch_bounds__(kCBT) k_evp_iter(int it, int n, const EvpParam
   if (i < n && !(masked && B.conv[i])) {
     const int dst = it & 1;
     const long long ln = n;
-    double sig[6];
-    double* g = gsh + threadIdx.x;
+    double sig[6], g;
     load_point<NS>(q, B, n, i, it, sig, g);
     double sig_new[6], ep[6], S[21], diff, d;
     evp_iterate<NS, IP>(q, sig, g, B.eps_c + i, B.ep_old + i, ln, sig_new, ep, S, nullptr, diff, d);
@@ -513,7 +526,6 @@ __global__ void __launch_bounds__(kCBT) k_evp_outputs(int n, const int* idx, lon
                                                        K* tangent) {
   if (ctrl->fail || !ctrl->done) return;
   __shared__ EvpParams<NS> q;
-  __shared__ double gsh[NS * kCBT];
   __shared__ double dgsh[NS * kCBT];
   load_params(&q, qg);
   __syncthreads();
@@ -522,31 +534,31 @@ __global__ void __launch_bounds__(kCBT) k_evp_outputs(int n, const int* idx, lon
   const long long ln = n;
   double sig[6];
   double sig_new[6], ep[6], S[21], Sinv[21], diff, d;
-  double* g = gsh + threadIdx.x;
+  double g;
   double* dgam = dgsh + threadIdx.x;
   load_point<NS>(q, B, n, i, B.last[i], sig, g);
   evp_iterate<NS, IP>(q, sig, g, B.eps_c + i, B.ep_old + i, ln, sig_new, ep, S, dgam, diff, d);
-  spd_inverse6(S, Sinv);
+  // trial state first (frees its registers before the rotations)
   const double acc_old = B.acc_old[i];
-  evp_harden<NS>(q, B.g_old + i, ln, acc_old, d, B.g_pend + i, ln);
-  const long long p = idx[i];
-  double Rm[36], vs[6], Ds[21];
-  load_rotation(rot9, npts, p, Rm);
-  to_specimen(Rm, sig_new, vs);
-#pragma unroll
-  for (int c = 0; c < 6; ++c) sig_out[c * npts + p] = vs[c];
-  tangent_to_specimen(Rm, Sinv, Ds);
-  store_tangent(tangent, npts, p, Ds);
-  double asum = 0.0;
+  B.g_pend[i] = evp_harden<NS>(q, B.g_old[i], acc_old, d);
 #pragma unroll
   for (int c = 0; c < 6; ++c) B.ep_pend[c * ln + i] = ep[c];
-#pragma unroll
+  double asum = 0.0;
+#pragma unroll 1
   for (int s = 0; s < NS; ++s) {
     const double dg = dgam[s * kCBT];
     B.gam_pend[s * ln + i] = B.gam_old[s * ln + i] + dg;
     asum += fabs(dg);
   }
   B.acc_pend[i] = acc_old + asum;
+  spd_inverse6(S, Sinv);
+  const long long p = idx[i];
+  double Rm[36], vs[6];
+  load_rotation(rot9, npts, p, Rm);
+  to_specimen(Rm, sig_new, vs);
+#pragma unroll
+  for (int c = 0; c < 6; ++c) sig_out[c * npts + p] = vs[c];
+  store_tangent_to_specimen(Rm, Sinv, tangent, npts, p);
 }
 
 
@@ -685,15 +697,15 @@ class EvpModel final : public Model<K> {
     MTL_HIP(hipMalloc(&B_.conv, n));
     MTL_HIP(hipMalloc(&B_.last, n * sizeof(int)));
     alloc(B_.ep_old, 6 * n);
-    alloc(B_.g_old, NS * n);
+    alloc(B_.g_old, n);
     alloc(B_.gam_old, NS * n);
     alloc(B_.acc_old, n);
     alloc(B_.ep_pend, 6 * n);
-    alloc(B_.g_pend, NS * n);
+    alloc(B_.g_pend, n);
     alloc(B_.gam_pend, NS * n);
     alloc(B_.acc_pend, n);
     alloc(ep_tr_, 6 * n);
-    alloc(g_tr_, NS * n);
+    alloc(g_tr_, n);
     alloc(gam_tr_, NS * n);
     alloc(acc_tr_, n);
     MTL_HIP(hipMalloc(&ctrl_, sizeof(EvpControl)));
@@ -721,8 +733,8 @@ class EvpModel final : public Model<K> {
     if (n == 0) return;
     fill(B_.ep_old, 6 * n, 0.0, s);
     fill(ep_tr_, 6 * n, 0.0, s);
-    fill(B_.g_old, NS * n, d_.g0, s);
-    fill(g_tr_, NS * n, d_.g0, s);
+    fill(B_.g_old, n, d_.g0, s);
+    fill(g_tr_, n, d_.g0, s);
     fill(B_.gam_old, NS * n, 0.0, s);
     fill(gam_tr_, NS * n, 0.0, s);
     fill(B_.acc_old, n, 0.0, s);
@@ -790,7 +802,7 @@ class EvpModel final : public Model<K> {
     const size_t n = size_t(this->n_);
     if (n == 0) return;
     MTL_HIP(hipMemcpyAsync(B_.ep_old, ep_tr_, 6 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    MTL_HIP(hipMemcpyAsync(B_.g_old, g_tr_, NS * n * sizeof(double), hipMemcpyDeviceToDevice, s));
+    MTL_HIP(hipMemcpyAsync(B_.g_old, g_tr_, n * sizeof(double), hipMemcpyDeviceToDevice, s));
     MTL_HIP(hipMemcpyAsync(B_.gam_old, gam_tr_, NS * n * sizeof(double), hipMemcpyDeviceToDevice, s));
     MTL_HIP(hipMemcpyAsync(B_.acc_old, acc_tr_, n * sizeof(double), hipMemcpyDeviceToDevice, s));
   }
@@ -825,7 +837,16 @@ class EvpModel final : public Model<K> {
         ncomp = 6;
         return true;
       }
-      case kFieldSlipResistance: fetch(g_tr_, NS); return true;
+      case kFieldSlipResistance: {
+        // one value per point on the device, the same for every system
+        fetch(g_tr_, 1);
+        std::vector<double> per_point(out);
+        out.resize(size_t(NS) * n);
+        for (int i = 0; i < n; ++i)
+          for (int k = 0; k < NS; ++k) out[size_t(i) * NS + k] = per_point[size_t(i)];
+        ncomp = NS;
+        return true;
+      }
       case kFieldSlipShear: fetch(gam_tr_, NS); return true;
       case kFieldAccumulatedSlip: fetch(acc_tr_, 1); return true;
       default: return false;

@@ -39,13 +39,14 @@ import numpy as np
 #   tangent and committed tangent: 2 x 21 packed of the Krylov type
 _FIELDS = 7 * 6 * 8 + 9 * 8
 # per point of a model: elastic index; EVP iteration, committed, pending and
-# trial state (EvpModel in csrc/constitutive.hip): per slip system g and
-# gamma for committed, pending and trial states; per point 42 doubles (crystal
-# strain, |guess|, 2 stresses, 2 slip increments, 3 x (plastic strain,
-# accumulated slip)), a flag, the last iteration and the point index
+# trial state (EvpModel in csrc/constitutive.hip): per slip system the slip
+# shear of the committed, pending and trial states; per point 45 doubles
+# (crystal strain, |guess|, 2 stresses, 2 slip increments, 3 x (plastic
+# strain, accumulated slip, slip resistance)), a flag, the last iteration and
+# the point index
 _ELASTIC_POINT = 4
-_EVP_POINT_PER_SYSTEM = 6 * 8
-_EVP_POINT_FIXED = 42 * 8 + 1 + 4 + 4
+_EVP_POINT_PER_SYSTEM = 3 * 8
+_EVP_POINT_FIXED = 45 * 8 + 1 + 4 + 4
 
 
 def footprint_bytes(dims, precision="mixed", evp_points=0, num_slip_systems=12):
