@@ -196,6 +196,7 @@ __global__ void __launch_bounds__(kCBT) k_elastic(int n, const int* idx, long lo
 template <int NS>
 struct EvpParams {
   double P[NS][6];
+  double PP[NS][21];  // P_s (x) P_s packed (the reference's schmid_outer_schmid)
   double Cinv[21];
   double gamma0, n, dt, ref_modulus, tol;
   int nint;  // n-1 when it is an integer in [0, 64], else -1
@@ -311,14 +312,15 @@ __device__ __forceinline__ void spd_inverse6(const double L[21], double Sinv[21]
 // at dgam[s * kCBT]) rather than registers, so the loop over systems need not
 // be unrolled and the kernel keeps enough waves resident to overlap its HBM
 // traffic with the FP64 arithmetic.  On return S holds the
-// Cholesky factor of C^-1 + dt M, sig_new the updated stress, ep the plastic
-// strain, d the slip increment dt sum_s |rate_s|, and dgam (when non-null) the
-// slip increments rate_s dt.  The total and committed plastic strains are read
-// (components strided by ln) only after the loop over systems, which keeps
-// them out of the registers live during it.
+// Cholesky factor of C^-1 + dt M, sig_new the updated stress, d the slip
+// increment dt sum_s |rate_s|, dgam (when non-null) the slip increments
+// rate_s dt, and ep (when ep_old_p is non-null) the plastic strain.  e0 is the
+// crystal strain minus the committed plastic strain (fixed during a call, set
+// up once per call), read (components strided by ln) only after the loop over
+// systems, which keeps it out of the registers live during it.
 template <int NS, bool IP>
 __device__ __forceinline__ void evp_iterate(const EvpParams<NS>& q, const double sig[6],
-                                            const double gs, const double* eps_p,
+                                            const double gs, const double* e0_p,
                                             const double* ep_old_p, long long ln,
                                             double sig_new[6], double ep[6], double S[21],
                                             double* dgam, double& diff, double& d) {
@@ -339,14 +341,14 @@ __device__ __forceinline__ void evp_iterate(const EvpParams<NS>& q, const double
 #pragma unroll
     for (int m = 0; m < 6; ++m) epr[m] += rate * q.P[s][m];
 #pragma unroll
-    for (int a = 0; a < 6; ++a)
-#pragma unroll
-      for (int b = a; b < 6; ++b) S[sym6(a, b)] += w * q.P[s][a] * q.P[s][b];
+    for (int k = 0; k < 21; ++k) S[k] += w * q.PP[s][k];
     if (dgam != nullptr) dgam[s * kCBT] = rate * q.dt;
     dsum += fabs(rate);
   }
+  if (ep_old_p != nullptr) {
 #pragma unroll
-  for (int m = 0; m < 6; ++m) ep[m] = ep_old_p[m * ln] + epr[m] * q.dt;
+    for (int m = 0; m < 6; ++m) ep[m] = ep_old_p[m * ln] + epr[m] * q.dt;
+  }
 #pragma unroll
   for (int k = 0; k < 21; ++k) S[k] = q.Cinv[k] + q.dt * S[k];
   double rc[6];
@@ -355,7 +357,7 @@ __device__ __forceinline__ void evp_iterate(const EvpParams<NS>& q, const double
     double acc = 0.0;
 #pragma unroll
     for (int b = 0; b < 6; ++b) acc += q.Cinv[sym6(a, b)] * sig[b];
-    rc[a] = acc - (eps_p[a * ln] - ep[a]);
+    rc[a] = acc - (e0_p[a * ln] - epr[a] * q.dt);
   }
   spd_factor6(S);
   double x[6];
@@ -399,7 +401,7 @@ __device__ __forceinline__ bool evp_converged(const EvpParams<NS>& q, double dif
 }
 
 struct EvpBuffers {
-  double* eps_c;     // [6][n]
+  double* e0;        // [6][n] crystal strain minus committed plastic strain
   double* gnorm;     // [n]
   double* sig[2];    // [6][n] ping-pong
   double* dsl[2];    // [n] ping-pong slip increment d of the iteration
@@ -429,7 +431,8 @@ __global__ void __launch_bounds__(kCBT) k_evp_setup(int n, const int* idx, long 
   for (int c = 0; c < 6; ++c) v[c] = eps[c * npts + p];
   to_crystal(Rm, v, vc);
 #pragma unroll
-  for (int c = 0; c < 6; ++c) B.eps_c[c * (long long)n + i] = vc[c];
+  for (int c = 0; c < 6; ++c)
+    B.e0[c * (long long)n + i] = vc[c] - B.ep_old[c * (long long)n + i];
   double nrm = 0.0;
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
@@ -464,8 +467,13 @@ __device__ __forceinline__ void load_point(const EvpParams<NS>& q, const EvpBuff
 // before the masked phase, only unconverged ones after it
 // (elastic_viscoplastic.py:116-158).  A point's state after iteration j lives
 // in ping-pong buffer j&1; last[i] records the last iteration it ran.
+// Workgroups per CU the iteration kernel is compiled for: four (128 VGPRs) with
+// the integer power, which fits without spilling for every instantiated system
+// count except 9; three otherwise.
+constexpr int evp_iter_min_blocks(int ns, bool ip) { return ip && ns != 9 ? 4 : 3; }
+
 template <int NS, bool IP>
-__global__ void __launch_bounds__(kCBT) k_evp_iter(int it, int n, const EvpParams<NS>* qg,
+__global__ void __launch_bounds__(kCBT, evp_iter_min_blocks(NS, IP)) k_evp_iter(int it, int n, const EvpParams<NS>* qg,
                                                     EvpBuffers B, EvpControl* ctrl) {
   if (ctrl->done) return;
   const int masked = ctrl->masked;
@@ -481,7 +489,7 @@ __global__ void __launch_bounds__(kCBT) k_evp_iter(int it, int n, const EvpParam
     double sig[6], g;
     load_point<NS>(q, B, n, i, it, sig, g);
     double sig_new[6], ep[6], S[21], diff, d;
-    evp_iterate<NS, IP>(q, sig, g, B.eps_c + i, B.ep_old + i, ln, sig_new, ep, S, nullptr, diff, d);
+    evp_iterate<NS, IP>(q, sig, g, B.e0 + i, nullptr, ln, sig_new, ep, S, nullptr, diff, d);
 #pragma unroll
     for (int c = 0; c < 6; ++c) B.sig[dst][c * ln + i] = sig_new[c];
     B.dsl[dst][i] = d;
@@ -537,7 +545,7 @@ __global__ void __launch_bounds__(kCBT) k_evp_outputs(int n, const int* idx, lon
   double g;
   double* dgam = dgsh + threadIdx.x;
   load_point<NS>(q, B, n, i, B.last[i], sig, g);
-  evp_iterate<NS, IP>(q, sig, g, B.eps_c + i, B.ep_old + i, ln, sig_new, ep, S, dgam, diff, d);
+  evp_iterate<NS, IP>(q, sig, g, B.e0 + i, B.ep_old + i, ln, sig_new, ep, S, dgam, diff, d);
   // trial state first (frees its registers before the rotations)
   const double acc_old = B.acc_old[i];
   B.g_pend[i] = evp_harden<NS>(q, B.g_old[i], acc_old, d);
@@ -679,6 +687,9 @@ class EvpModel final : public Model<K> {
     for (int k = 0; k < 21; ++k) q_.Cinv[k] = ci.c[k];
     for (int s = 0; s < NS; ++s)
       for (int m = 0; m < 6; ++m) q_.P[s][m] = d.schmid[s * 6 + m];
+    for (int s = 0; s < NS; ++s)
+      for (int a = 0; a < 6; ++a)
+        for (int b = a; b < 6; ++b) q_.PP[s][sym6(a, b)] = q_.P[s][a] * q_.P[s][b];
     q_.gamma0 = d.gamma0;
     q_.n = d.n;
     const double e = d.n - 1.0;
@@ -688,7 +699,7 @@ class EvpModel final : public Model<K> {
     for (int k = 0; k < 4; ++k) q_.hp[k] = d.hp[k];
     const long long n = std::max(this->n_, 1);
     auto alloc = [&](double*& ptr, long long cnt) { MTL_HIP(hipMalloc(&ptr, cnt * sizeof(double))); };
-    alloc(B_.eps_c, 6 * n);
+    alloc(B_.e0, 6 * n);
     alloc(B_.gnorm, n);
     for (int k = 0; k < 2; ++k) {
       alloc(B_.sig[k], 6 * n);
@@ -714,7 +725,7 @@ class EvpModel final : public Model<K> {
     MTL_HIP(hipHostMalloc(&hctrl_, sizeof(EvpControl), hipHostMallocDefault));
   }
   ~EvpModel() override {
-    for (double* p : {B_.eps_c, B_.gnorm, B_.sig[0], B_.sig[1], B_.dsl[0], B_.dsl[1], B_.ep_old,
+    for (double* p : {B_.e0, B_.gnorm, B_.sig[0], B_.sig[1], B_.dsl[0], B_.dsl[1], B_.ep_old,
                       B_.g_old, B_.gam_old, B_.acc_old, B_.ep_pend, B_.g_pend, B_.gam_pend,
                       B_.acc_pend, ep_tr_, g_tr_, gam_tr_, acc_tr_})
       if (p) (void)hipFree(p);
