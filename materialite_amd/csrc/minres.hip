@@ -22,13 +22,14 @@ namespace mtl {
 namespace {
 
 constexpr int kVBT = 256;      // vector kernels
+constexpr int kSBT = 1024;     // one-workgroup scalar kernels (up to 8192 partials to sum)
 constexpr int kVBlocks = 2048;  // grid-stride blocks for vector kernels
 constexpr double kEps = DBL_EPSILON;
 
 __device__ double sum_partials(const double* partials, int count, double* red) {
   double acc = 0.0;
-  for (int i = threadIdx.x; i < count; i += blockDim.x) acc += partials[i];
-  return block_sum<kVBT>(acc, red);
+  for (int i = threadIdx.x; i < count; i += kSBT) acc += partials[i];
+  return block_sum<kSBT>(acc, red);
 }
 
 // r1 is zeroed although iteration 1 multiplies it by 0 (c_r1 = 0): stale
@@ -52,9 +53,9 @@ __global__ void __launch_bounds__(kVBT) k_init(long long n6, const double* b, T*
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-__global__ void __launch_bounds__(kVBT) k_start(MinresState* st, const double* partials, int np,
+__global__ void __launch_bounds__(kSBT) k_start(MinresState* st, const double* partials, int np,
                                                  double rtol, long long maxiter) {
-  __shared__ double red[kVBT / 64];
+  __shared__ double red[kSBT / 64];
   const double b2 = sum_partials(partials, np, red);
   if (threadIdx.x != 0) return;
   MinresState z{};
@@ -78,8 +79,8 @@ __global__ void __launch_bounds__(kVBT) k_start(MinresState* st, const double* p
   *st = z;
 }
 
-__global__ void __launch_bounds__(kVBT) k_alpha(MinresState* st, const double* partials, int np) {
-  __shared__ double red[kVBT / 64];
+__global__ void __launch_bounds__(kSBT) k_alpha(MinresState* st, const double* partials, int np) {
+  __shared__ double red[kSBT / 64];
   if (st->done) return;
   const double a = sum_partials(partials, np, red);
   if (threadIdx.x != 0) return;
@@ -104,8 +105,8 @@ __global__ void __launch_bounds__(kVBT) k_lanczos(long long n6, const MinresStat
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-__global__ void __launch_bounds__(kVBT) k_givens(MinresState* st, const double* partials, int np) {
-  __shared__ double red[kVBT / 64];
+__global__ void __launch_bounds__(kSBT) k_givens(MinresState* st, const double* partials, int np) {
+  __shared__ double red[kSBT / 64];
   if (st->done) return;
   const double b2 = sum_partials(partials, np, red);
   if (threadIdx.x != 0) return;
@@ -163,8 +164,8 @@ __global__ void __launch_bounds__(kVBT) k_update(long long n6, const MinresState
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-__global__ void __launch_bounds__(kVBT) k_tests(MinresState* st, const double* partials, int np) {
-  __shared__ double red[kVBT / 64];
+__global__ void __launch_bounds__(kSBT) k_tests(MinresState* st, const double* partials, int np) {
+  __shared__ double red[kSBT / 64];
   if (st->done) return;
   const double x2 = sum_partials(partials, np, red);
   if (threadIdx.x != 0) return;
@@ -229,7 +230,7 @@ int Minres<T>::solve(const double* b, const T* tangent, const Vec6& mask, double
   const int vb = vec_blocks(n6_);
   hipLaunchKernelGGL(k_init<T>, dim3(vb), dim3(kVBT), 0, s, n6_, b, r1, r2, x_, w2, wf, partials_);
   MTL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_start, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb, rtol, maxiter);
+  hipLaunchKernelGGL(k_start, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb, rtol, maxiter);
   MTL_LAUNCH_CHECK();
   // at itn 1 the previous w's are zero: w1 := wf (zeroed), w2 (zeroed)
   std::swap(w1, wf);
@@ -241,18 +242,18 @@ int Minres<T>::solve(const double* b, const T* tangent, const Vec6& mask, double
   while (true) {
     for (int it = 0; it < chunk; ++it) {
       const int np = sp_.matvec_minres(r2, tangent, r1, mask, state_, partials_, s);
-      profiler().launch(kKMinresScalar, k_alpha, dim3(1), dim3(kVBT), 0, s, state_, partials_, np);
+      profiler().launch(kKMinresScalar, k_alpha, dim3(1), dim3(kSBT), 0, s, state_, partials_, np);
       MTL_LAUNCH_CHECK();
       profiler().launch(kKMinresVector, k_lanczos<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, r2, partials_);
       MTL_LAUNCH_CHECK();
-      profiler().launch(kKMinresScalar, k_givens, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb);
+      profiler().launch(kKMinresScalar, k_givens, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb);
       MTL_LAUNCH_CHECK();
       // Lanczos roles rotate: r1 <- r2 (old), r2 <- y (stored in old r1)
       std::swap(r1, r2);
       // v_k = r2_old / beta_k now lives in r1
       profiler().launch(kKMinresVector, k_update<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, w1, w2, wf, x_, partials_);
       MTL_LAUNCH_CHECK();
-      profiler().launch(kKMinresScalar, k_tests, dim3(1), dim3(kVBT), 0, s, state_, partials_, vb);
+      profiler().launch(kKMinresScalar, k_tests, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb);
       MTL_LAUNCH_CHECK();
       // w roles rotate: w1 <- w2, w2 <- new w, free slot <- old w1
       T* old_w1 = w1;
