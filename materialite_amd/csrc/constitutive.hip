@@ -330,12 +330,16 @@ __device__ __forceinline__ void evp_iterate(const EvpParams<NS>& q, const double
 #pragma unroll
   for (int k = 0; k < 21; ++k) S[k] = 0.0;
   double dsum = 0.0;
+  // one reciprocal for all systems (the slip resistance is the same for every
+  // system of a point) instead of two FP64 divisions per system; the rate
+  // differs from the divided form by a few ulps
+  const double ginv = 1.0 / gs;
 #pragma unroll 1
   for (int s = 0; s < NS; ++s) {
     double tau = 0.0;
 #pragma unroll
     for (int m = 0; m < 6; ++m) tau += q.P[s][m] * sig[m];
-    const double pref = q.gamma0 * powi_abs<IP>(tau / gs, q.nint, q.n - 1.0) / gs;
+    const double pref = q.gamma0 * powi_abs<IP>(tau * ginv, q.nint, q.n - 1.0) * ginv;
     const double rate = pref * tau;
     const double w = q.n * pref;
 #pragma unroll
