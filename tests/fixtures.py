@@ -53,9 +53,9 @@ def voronoi_8():
     return rot, pore, regions
 
 
-def evp_model(hardening, props):
+def evp_model(hardening, props, rate_exponent=10.0):
     schmid = T.schmid_mandel(*T.octahedral_slip_systems())
-    return C.ElasticViscoplastic(TI_STIFFNESS, schmid, 1.0, 10.0, 300.0, hardening, props)
+    return C.ElasticViscoplastic(TI_STIFFNESS, schmid, 1.0, rate_exponent, 300.0, hardening, props)
 
 
 # J2 plasticity cases of tests/test_isotropic_plasticity.py:19-140 (8^3, identity

@@ -23,8 +23,13 @@ def _need_gpu():
         pytest.fail("GPU tests need a HIP device")
 
 
+# The long-x grids (nx >= 250) take the x pass that streams the components
+# through LDS one at a time (six resident components do not fit two kz
+# columns there), with full, partial (256 x 2 x 40: nzc = 20 in tiles of 8) and
+# odd-width (256 x 4 x 6: nzc = 3) tiles and a radix-5 length (250).
 GRIDS = [(3, 4, 5), (4, 4, 4), (2, 3, 6), (1, 4, 3), (6, 5, 8), (8, 8, 8), (7, 9, 10),
-         (16, 16, 16), (12, 10, 14), (32, 32, 32), (48, 48, 48)]
+         (16, 16, 16), (12, 10, 14), (32, 32, 32), (48, 48, 48),
+         (256, 4, 6), (256, 2, 40), (250, 3, 4), (512, 2, 6)]
 
 
 @pytest.mark.parametrize("dims", GRIDS)
@@ -49,6 +54,41 @@ def test_projection_matches_oracle(dims, precision):
         scale = np.abs(ref).max()
         tol = 1e-13 if precision == "fp64" else 2e-6
         assert np.abs(got - ref).max() <= tol * scale, (dims, precision, np.abs(got - ref).max() / scale)
+
+
+@pytest.mark.parametrize("dims,precision", [((128, 128, 128), "fp64"), ((128, 128, 128), "mixed"),
+                                             ((256, 256, 256), "mixed")])
+def test_projection_properties_full_size(dims, precision):
+    """At the workload sizes (128^3, and 256^3 where the x pass streams the
+    components): P is an orthogonal projector -- P(P x) = P x and
+    <P x, w> = <x, P w> -- and P(a x + b w) = a P x + b P w."""
+    from materialite_amd._native import NativeSolver
+
+    print(f"projection properties {dims} {precision}", flush=True)
+    n = int(np.prod(dims))
+    gen = torch.Generator(device="cuda").manual_seed(sum(dims))
+    x = torch.randn(6, n, dtype=torch.float64, device="cuda", generator=gen)
+    w = torch.randn(6, n, dtype=torch.float64, device="cuda", generator=gen)
+    mask = np.array([0, 1, 1, 1, 1, 1.0])
+    sol = NativeSolver(dims, precision)
+
+    def proj(a):
+        out = torch.empty_like(a)
+        torch.cuda.synchronize()
+        sol.project_dev(a.data_ptr(), out.data_ptr(), mask)
+        torch.cuda.synchronize()
+        return out
+
+    px, pw = proj(x), proj(w)
+    ppx = proj(px)
+    pxw = proj(2.0 * x - 3.0 * w)
+    sol.close()
+    tol = 1e-12 if precision == "fp64" else 2e-6
+    scale = px.abs().max().item()
+    assert (ppx - px).abs().max().item() <= tol * scale
+    assert (pxw - (2.0 * px - 3.0 * pw)).abs().max().item() <= 5 * tol * scale
+    lhs, rhs = (px * w).sum().item(), (x * pw).sum().item()
+    assert abs(lhs - rhs) <= tol * 10 * (px.norm() * w.norm()).item()
 
 
 def _pore_case():
@@ -102,15 +142,17 @@ def test_minres_matches_oracle(precision):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "mixed"])
-def test_evp_update_matches_oracle(precision):
+@pytest.mark.parametrize("rate_exponent", [10.0, 10.5])
+def test_evp_update_matches_oracle(precision, rate_exponent):
+    """rate exponent 10: |tau/g|^9 by squaring; 10.5: the general pow()."""
     from materialite_amd._native import NativeSolver
     from oracle import tensors as T
     from tests import fixtures as F
 
-    print(f"evp update {precision}", flush=True)
+    print(f"evp update {precision} n={rate_exponent}", flush=True)
     rot, _, _ = F.voronoi_8()
     props = {"tau_1": 100.0, "theta_0": 20000.0, "theta_1": 10.0}
-    model = F.evp_model("voce", props)
+    model = F.evp_model("voce", props, rate_exponent)
     tangent = model.initialize(rot)
     rng = np.random.default_rng(5)
     strain = np.tile([-0.0012, -0.0012, 0.004, 0, 0, 0], (512, 1)) + 2e-4 * rng.normal(size=(512, 6))
@@ -119,7 +161,7 @@ def test_evp_update_matches_oracle(precision):
     assert it_ref
     sol = NativeSolver((8, 8, 8), precision)
     schmid = T.schmid_mandel(*T.octahedral_slip_systems())
-    sol.add_evp(F.TI_STIFFNESS, schmid, 1.0, 10.0, 300.0, 2,
+    sol.add_evp(F.TI_STIFFNESS, schmid, 1.0, rate_exponent, 300.0, 2,
                 [props["theta_0"], props["theta_1"], props["tau_1"]])
     sol.set_microstructure(rot, np.zeros(512, np.int32))
     sol.initialize()
