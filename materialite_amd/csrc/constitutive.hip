@@ -483,11 +483,15 @@ __global__ void __launch_bounds__(kCBT, evp_iter_min_blocks(NS, IP)) k_evp_iter(
   const int masked = ctrl->masked;
   __shared__ EvpParams<NS> q;
   __shared__ int cnt;
+  const int i = blockIdx.x * kCBT + threadIdx.x;
+  const bool active = i < n && !(masked && B.conv[i]);
+  // in the masked phase most workgroups have no point left: they skip the
+  // parameter load (a few KB per workgroup) and return
+  if (!__syncthreads_or(active)) return;
   load_params(&q, qg);
   if (threadIdx.x == 0) cnt = 0;
   __syncthreads();
-  const int i = blockIdx.x * kCBT + threadIdx.x;
-  if (i < n && !(masked && B.conv[i])) {
+  if (active) {
     const int dst = it & 1;
     const long long ln = n;
     double sig[6], g;
@@ -789,7 +793,9 @@ class EvpModel final : public Model<K> {
       MTL_HIP(hipMemcpyAsync(hctrl_, ctrl_, sizeof(EvpControl), hipMemcpyDeviceToHost, s));
       stream_sync(s);
       if (hctrl_->done || it >= kEvpMaxIterations) break;
-      chunk = 4;
+      // a host round trip costs about as much as 8 no-op iterations past
+      // convergence (workgroups with no unconverged point return at once)
+      chunk = 8;
     }
     if (std::getenv("MTL_DEBUG") != nullptr) {
       std::fprintf(stderr, "[mtl] evp n=%d converging per iteration:", n);
