@@ -678,6 +678,19 @@ class ElasticModel final : public Model<K> {
   Stiff21 C_;
 };
 
+// Three buffer sets for a model's internal state: the committed set (old),
+// the trial set (the last successful call's outputs, tr) and a free set the
+// next call writes (pend).  Adopting a call's outputs and committing relabel
+// sets instead of copying them; after a commit old and tr name the same set.
+struct SlotRing {
+  int old = 0, tr = 0, pend = 1;
+  void adopt() {  // the call's outputs become the trial state
+    tr = pend;
+    pend = 3 - old - tr;  // the set that is neither committed nor trial
+  }
+  void commit() { old = tr; }
+};
+
 template <typename K, int NS>
 class EvpModel final : public Model<K> {
  public:
@@ -715,28 +728,24 @@ class EvpModel final : public Model<K> {
     }
     MTL_HIP(hipMalloc(&B_.conv, n));
     MTL_HIP(hipMalloc(&B_.last, n * sizeof(int)));
-    alloc(B_.ep_old, 6 * n);
-    alloc(B_.g_old, n);
-    alloc(B_.gam_old, NS * n);
-    alloc(B_.acc_old, n);
-    alloc(B_.ep_pend, 6 * n);
-    alloc(B_.g_pend, n);
-    alloc(B_.gam_pend, NS * n);
-    alloc(B_.acc_pend, n);
-    alloc(ep_tr_, 6 * n);
-    alloc(g_tr_, n);
-    alloc(gam_tr_, NS * n);
-    alloc(acc_tr_, n);
+    for (auto& st : slot_) {
+      alloc(st.ep, 6 * n);
+      alloc(st.g, n);
+      alloc(st.gam, NS * n);
+      alloc(st.acc, n);
+    }
+    bind();
     MTL_HIP(hipMalloc(&ctrl_, sizeof(EvpControl)));
     MTL_HIP(hipMalloc(&qd_, sizeof(EvpParams<NS>)));
     MTL_HIP(hipHostMalloc(&qh_, sizeof(EvpParams<NS>), hipHostMallocDefault));
     MTL_HIP(hipHostMalloc(&hctrl_, sizeof(EvpControl), hipHostMallocDefault));
   }
   ~EvpModel() override {
-    for (double* p : {B_.e0, B_.gnorm, B_.sig[0], B_.sig[1], B_.dsl[0], B_.dsl[1], B_.ep_old,
-                      B_.g_old, B_.gam_old, B_.acc_old, B_.ep_pend, B_.g_pend, B_.gam_pend,
-                      B_.acc_pend, ep_tr_, g_tr_, gam_tr_, acc_tr_})
+    for (double* p : {B_.e0, B_.gnorm, B_.sig[0], B_.sig[1], B_.dsl[0], B_.dsl[1]})
       if (p) (void)hipFree(p);
+    for (auto& st : slot_)
+      for (double* p : {st.ep, st.g, st.gam, st.acc})
+        if (p) (void)hipFree(p);
     if (B_.conv) (void)hipFree(B_.conv);
     if (B_.last) (void)hipFree(B_.last);
     if (ctrl_) (void)hipFree(ctrl_);
@@ -750,14 +759,13 @@ class EvpModel final : public Model<K> {
     // elastic_viscoplastic.py:52-89: zero plastic state, g = g0
     const long long n = this->n_;
     if (n == 0) return;
-    fill(B_.ep_old, 6 * n, 0.0, s);
-    fill(ep_tr_, 6 * n, 0.0, s);
-    fill(B_.g_old, n, d_.g0, s);
-    fill(g_tr_, n, d_.g0, s);
-    fill(B_.gam_old, NS * n, 0.0, s);
-    fill(gam_tr_, NS * n, 0.0, s);
-    fill(B_.acc_old, n, 0.0, s);
-    fill(acc_tr_, n, 0.0, s);
+    ring_ = SlotRing{};
+    bind();
+    const EvpState& st = slot_[ring_.old];  // committed and trial state
+    fill(st.ep, 6 * n, 0.0, s);
+    fill(st.g, n, d_.g0, s);
+    fill(st.gam, NS * n, 0.0, s);
+    fill(st.acc, n, 0.0, s);
     hipLaunchKernelGGL(k_elastic<K>, dim3(ceil_div(this->n_, kCBT)), dim3(kCBT), 0, s, this->n_,
                        this->idx_, this->npts_, rot9, C_, nullptr, nullptr, tangent, 0);
     MTL_LAUNCH_CHECK();
@@ -771,7 +779,7 @@ class EvpModel final : public Model<K> {
     *qh_ = q_;
     MTL_HIP(hipMemcpyAsync(qd_, qh_, sizeof(EvpParams<NS>), hipMemcpyHostToDevice, s));
     const int blocks = ceil_div(n, kCBT);
-    B_.g_trial = g_tr_;
+    bind();
     hipLaunchKernelGGL(k_evp_setup<NS>, dim3(blocks), dim3(kCBT), 0, s, n, this->idx_, this->npts_,
                        rot9, eps, guess, B_);
     MTL_LAUNCH_CHECK();
@@ -811,21 +819,15 @@ class EvpModel final : public Model<K> {
                       ctrl_, sig, tangent);
     MTL_LAUNCH_CHECK();
     // adopt the call's outputs as the trial state (elastic_viscoplastic.py:175-182)
-    std::swap(B_.ep_pend, ep_tr_);
-    std::swap(B_.g_pend, g_tr_);
-    std::swap(B_.gam_pend, gam_tr_);
-    std::swap(B_.acc_pend, acc_tr_);
+    ring_.adopt();
+    bind();
     return hctrl_->iterations;
   }
 
-  void commit(const double*, hipStream_t s) override {
-    // elastic_viscoplastic.py:255-271
-    const size_t n = size_t(this->n_);
-    if (n == 0) return;
-    MTL_HIP(hipMemcpyAsync(B_.ep_old, ep_tr_, 6 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    MTL_HIP(hipMemcpyAsync(B_.g_old, g_tr_, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    MTL_HIP(hipMemcpyAsync(B_.gam_old, gam_tr_, NS * n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    MTL_HIP(hipMemcpyAsync(B_.acc_old, acc_tr_, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  void commit(const double*, hipStream_t) override {
+    // elastic_viscoplastic.py:255-271: the trial state becomes the committed one
+    ring_.commit();
+    bind();
   }
 
   bool field(int which, const double* rot9, std::vector<double>& out, int& ncomp,
@@ -849,7 +851,7 @@ class EvpModel final : public Model<K> {
         MTL_HIP(hipMalloc(&tmp, std::max(6 * n, 1) * sizeof(double)));
         if (n)
           hipLaunchKernelGGL(k_ep_specimen, dim3(ceil_div(n, kCBT)), dim3(kCBT), 0, s, n,
-                             this->idx_, this->npts_, rot9, B_.ep_old, tmp);
+                             this->idx_, this->npts_, rot9, slot_[ring_.old].ep, tmp);
         out.resize(size_t(6) * n);
         if (n) MTL_HIP(hipMemcpyAsync(out.data(), tmp, out.size() * sizeof(double),
                                       hipMemcpyDeviceToHost, s));
@@ -860,7 +862,7 @@ class EvpModel final : public Model<K> {
       }
       case kFieldSlipResistance: {
         // one value per point on the device, the same for every system
-        fetch(g_tr_, 1);
+        fetch(slot_[ring_.tr].g, 1);
         std::vector<double> per_point(out);
         out.resize(size_t(NS) * n);
         for (int i = 0; i < n; ++i)
@@ -868,8 +870,8 @@ class EvpModel final : public Model<K> {
         ncomp = NS;
         return true;
       }
-      case kFieldSlipShear: fetch(gam_tr_, NS); return true;
-      case kFieldAccumulatedSlip: fetch(acc_tr_, 1); return true;
+      case kFieldSlipShear: fetch(slot_[ring_.tr].gam, NS); return true;
+      case kFieldAccumulatedSlip: fetch(slot_[ring_.tr].acc, 1); return true;
       default: return false;
     }
   }
@@ -879,7 +881,24 @@ class EvpModel final : public Model<K> {
   Stiff21 C_;
   EvpParams<NS> q_{};
   EvpBuffers B_{};
-  double *ep_tr_ = nullptr, *g_tr_ = nullptr, *gam_tr_ = nullptr, *acc_tr_ = nullptr;
+  struct EvpState {
+    double *ep = nullptr, *g = nullptr, *gam = nullptr, *acc = nullptr;
+  };
+  EvpState slot_[3];
+  SlotRing ring_;
+  // the kernels' view of the committed, free and trial sets
+  void bind() {
+    const EvpState &o = slot_[ring_.old], &p = slot_[ring_.pend], &t = slot_[ring_.tr];
+    B_.ep_old = o.ep;
+    B_.g_old = o.g;
+    B_.gam_old = o.gam;
+    B_.acc_old = o.acc;
+    B_.ep_pend = p.ep;
+    B_.g_pend = p.g;
+    B_.gam_pend = p.gam;
+    B_.acc_pend = p.acc;
+    B_.g_trial = t.g;
+  }
   EvpControl* ctrl_ = nullptr;
   EvpControl* hctrl_ = nullptr;
   EvpParams<NS>* qd_ = nullptr;  // device copy read by the kernels
@@ -1099,24 +1118,21 @@ class J2Model final : public Model<K> {
     for (int k = 0; k < 4; ++k) q_.hp[k] = d.hp[k];
     const long long n = std::max(this->n_, 1);
     auto alloc = [&](double*& ptr, long long cnt) { MTL_HIP(hipMalloc(&ptr, cnt * sizeof(double))); };
-    alloc(B_.ep_old, 6 * n);
-    alloc(B_.eq_old, n);
-    alloc(B_.y_old, n);
-    alloc(B_.ep_pend, 6 * n);
-    alloc(B_.eq_pend, n);
-    alloc(B_.y_pend, n);
-    alloc(ep_tr_, 6 * n);
-    alloc(eq_tr_, n);
-    alloc(y_tr_, n);
+    for (auto& st : slot_) {
+      alloc(st.ep, 6 * n);
+      alloc(st.eq, n);
+      alloc(st.y, n);
+    }
+    bind();
     MTL_HIP(hipMalloc(&ctl_, sizeof(J2Control)));
     MTL_HIP(hipMalloc(&qd_, sizeof(J2Params)));
     MTL_HIP(hipHostMalloc(&qh_, sizeof(J2Params), hipHostMallocDefault));
     MTL_HIP(hipHostMalloc(&hctl_, sizeof(J2Control), hipHostMallocDefault));
   }
   ~J2Model() override {
-    for (double* p : {B_.ep_old, B_.eq_old, B_.y_old, B_.ep_pend, B_.eq_pend, B_.y_pend, ep_tr_,
-                      eq_tr_, y_tr_})
-      if (p) (void)hipFree(p);
+    for (auto& st : slot_)
+      for (double* p : {st.ep, st.eq, st.y})
+        if (p) (void)hipFree(p);
     if (ctl_) (void)hipFree(ctl_);
     if (qd_) (void)hipFree(qd_);
     if (qh_) (void)hipHostFree(qh_);
@@ -1128,9 +1144,12 @@ class J2Model final : public Model<K> {
     // isotropic_elastic_plastic.py:55-75: zero plastic state, yield stress Y0
     const long long n = this->n_;
     if (n == 0) return;
-    for (double* p : {B_.ep_old, ep_tr_}) fill(p, 6 * n, 0.0, s);
-    for (double* p : {B_.eq_old, eq_tr_}) fill(p, n, 0.0, s);
-    for (double* p : {B_.y_old, y_tr_}) fill(p, n, d_.g0, s);
+    ring_ = SlotRing{};
+    bind();
+    const J2State& st = slot_[ring_.old];  // committed and trial state
+    fill(st.ep, 6 * n, 0.0, s);
+    fill(st.eq, n, 0.0, s);
+    fill(st.y, n, d_.g0, s);
     hipLaunchKernelGGL(k_const_tangent<K>, dim3(ceil_div(this->n_, kCBT)), dim3(kCBT), 0, s,
                        this->n_, this->idx_, this->npts_, pack21(d_.C), tangent);
     MTL_LAUNCH_CHECK();
@@ -1164,20 +1183,16 @@ class J2Model final : public Model<K> {
     MTL_LAUNCH_CHECK();
     if (hctl_->plastic) {
       // the call's state becomes the trial state (isotropic_elastic_plastic.py:131-137)
-      std::swap(B_.ep_pend, ep_tr_);
-      std::swap(B_.eq_pend, eq_tr_);
-      std::swap(B_.y_pend, y_tr_);
+      ring_.adopt();
+      bind();
     }
     return iters;
   }
 
-  void commit(const double*, hipStream_t s) override {
-    // isotropic_elastic_plastic.py:170-179
-    const size_t n = size_t(this->n_);
-    if (n == 0) return;
-    MTL_HIP(hipMemcpyAsync(B_.ep_old, ep_tr_, 6 * n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    MTL_HIP(hipMemcpyAsync(B_.eq_old, eq_tr_, n * sizeof(double), hipMemcpyDeviceToDevice, s));
-    MTL_HIP(hipMemcpyAsync(B_.y_old, y_tr_, n * sizeof(double), hipMemcpyDeviceToDevice, s));
+  void commit(const double*, hipStream_t) override {
+    // isotropic_elastic_plastic.py:170-179: the trial state becomes the committed one
+    ring_.commit();
+    bind();
   }
 
   bool field(int which, const double*, std::vector<double>& out, int& ncomp,
@@ -1194,9 +1209,9 @@ class J2Model final : public Model<K> {
       ncomp = nc;
     };
     switch (which) {
-      case kFieldPlasticStrain: fetch(ep_tr_, 6); return true;
-      case kFieldEqPlasticStrain: fetch(eq_tr_, 1); return true;
-      case kFieldYieldStress: fetch(y_tr_, 1); return true;
+      case kFieldPlasticStrain: fetch(slot_[ring_.tr].ep, 6); return true;
+      case kFieldEqPlasticStrain: fetch(slot_[ring_.tr].eq, 1); return true;
+      case kFieldYieldStress: fetch(slot_[ring_.tr].y, 1); return true;
       default: return false;
     }
   }
@@ -1205,7 +1220,21 @@ class J2Model final : public Model<K> {
   ModelDesc d_;
   J2Params q_{};
   J2Buffers B_{};
-  double *ep_tr_ = nullptr, *eq_tr_ = nullptr, *y_tr_ = nullptr;
+  struct J2State {
+    double *ep = nullptr, *eq = nullptr, *y = nullptr;
+  };
+  J2State slot_[3];
+  SlotRing ring_;
+  // the kernels' view of the committed and free sets
+  void bind() {
+    const J2State &o = slot_[ring_.old], &p = slot_[ring_.pend];
+    B_.ep_old = o.ep;
+    B_.eq_old = o.eq;
+    B_.y_old = o.y;
+    B_.ep_pend = p.ep;
+    B_.eq_pend = p.eq;
+    B_.y_pend = p.y;
+  }
   J2Control* ctl_ = nullptr;
   J2Control* hctl_ = nullptr;
   J2Params* qd_ = nullptr;

@@ -364,6 +364,7 @@ class Solver final : public SolverBase {
     hipLaunchKernelGGL(k_guess<K>, dim3(nb), dim3(kNBT), 0, s_, n_, P.time_step_id == 0 ? 1 : 0,
                        sig_old_, D_, ds, de, guess_);
     MTL_LAUNCH_CHECK();
+    guess_is_stress_ = false;
 
     const int kMaxNewton = 100;  // small_strain_fft.py:125
     const long long maxiter = 5LL * 6LL * n_;
@@ -392,7 +393,9 @@ class Solver final : public SolverBase {
         const Norms nm = norms(false);
         vm = von_mises_of_mean(nm.mean);
         eq_err = nm.maxb / vm;
-        copy6(guess_, sig_);
+        // the next constitutive call's guess is this stress; it rewrites sig_
+        std::swap(guess_, sig_);
+        guess_is_stress_ = true;
         max_constit = std::max(max_constit, it);
       }
       old_eq = std::max(eq_err, P.global_tolerance);
@@ -422,6 +425,8 @@ class Solver final : public SolverBase {
       MTL_LAUNCH_CHECK();
       const int it = constitutive_all(eps_, guess_, dt, sig_);
       if (!it) {
+        // sig_ is left as the last accepted stress (the failed call's guess)
+        if (guess_is_stress_) std::swap(guess_, sig_);
         restore_tangent();
         R.constitutive_failed = 1;
         break;
@@ -444,7 +449,10 @@ class Solver final : public SolverBase {
       if (debug_enabled())
         std::fprintf(stderr, "[mtl]   eq_err %.3e strain_err %.3e projected %.3e -> %s\n", eq_err,
                      strain_err, projected, converged ? "converged" : "continue");
-      copy6(guess_, sig_);
+      if (!converged && iteration < kMaxNewton) {
+        std::swap(guess_, sig_);  // as above: sig_ is rewritten before it is read
+        guess_is_stress_ = true;
+      }
       R.scaled_strain_error = strain_err;
       R.projected_scaled_strain_error = projected;
       if (iteration == kMaxNewton && !converged) break;
@@ -562,6 +570,9 @@ class Solver final : public SolverBase {
   double *eps_ = nullptr, *sig_ = nullptr, *sig_old_ = nullptr, *eps_old_ = nullptr,
          *fluct_ = nullptr, *guess_ = nullptr, *b_ = nullptr, *rot_ = nullptr;
   K *D_ = nullptr, *D_old_ = nullptr;
+  // guess_ holds the stress of the last accepted constitutive call (the
+  // buffers were swapped instead of copied) rather than the predictor's guess
+  bool guess_is_stress_ = false;
   double* partials_ = nullptr;
   double* hpart_ = nullptr;
   std::vector<ModelDesc> descs_;
