@@ -269,12 +269,14 @@ class Solver final : public SolverBase {
     MTL_HIP(hipMemsetAsync(D_, 0, 21 * n_ * sizeof(K), s_));
     for (auto& m : models_) m->init_tangent(rot_, D_, s_);
     MTL_HIP(hipMemcpyAsync(D_old_, D_, 21 * n_ * sizeof(K), hipMemcpyDeviceToDevice, s_));
+    sig_stale_ = eps_stale_ = d_stale_ = guess_is_stress_ = false;
     stream_sync(s_);
   }
 
   // Multiphase dispatch (multiphase.py:38-51): fails as soon as a model fails
   int constitutive_all(const double* eps, const double* guess, double dt, double* sig) {
     int iters = 0;
+    d_stale_ = false;  // D_ is rewritten (or restored on failure)
     for (auto& m : models_) {
       const int it = m->calculate(rot_, eps, guess, dt, sig, D_, s_);
       if (!it) return 0;
@@ -320,7 +322,7 @@ class Solver final : public SolverBase {
 
   void tangent_mean(double out36[36]) {
     const int nb = grid_for(n_);
-    hipLaunchKernelGGL(k_tangent_sum<K>, dim3(nb), dim3(kNBT), 0, s_, n_, D_, partials_);
+    hipLaunchKernelGGL(k_tangent_sum<K>, dim3(nb), dim3(kNBT), 0, s_, n_, cur_D(), partials_);
     MTL_LAUNCH_CHECK();
     MTL_HIP(hipMemcpyAsync(hpart_, partials_, nb * 21 * sizeof(double), hipMemcpyDeviceToHost, s_));
     stream_sync(s_);
@@ -361,8 +363,9 @@ class Solver final : public SolverBase {
     hipLaunchKernelGGL(k_predictor, dim3(nb), dim3(kNBT), 0, s_, n_, eps_old_, fluct_, eps_, de,
                        pred, P.reset_fluctuation);
     MTL_LAUNCH_CHECK();
+    eps_stale_ = false;
     hipLaunchKernelGGL(k_guess<K>, dim3(nb), dim3(kNBT), 0, s_, n_, P.time_step_id == 0 ? 1 : 0,
-                       sig_old_, D_, ds, de, guess_);
+                       sig_old_, cur_D(), ds, de, guess_);
     MTL_LAUNCH_CHECK();
     guess_is_stress_ = false;
 
@@ -384,6 +387,7 @@ class Solver final : public SolverBase {
       ++iteration;
       if (iteration == 1) {
         const int it = constitutive_all(eps_, guess_, dt, sig_);
+        sig_stale_ = false;
         if (!it) {
           restore_tangent();
           R.constitutive_failed = 1;
@@ -424,6 +428,7 @@ class Solver final : public SolverBase {
                          fluct_, shift);
       MTL_LAUNCH_CHECK();
       const int it = constitutive_all(eps_, guess_, dt, sig_);
+      sig_stale_ = false;
       if (!it) {
         // sig_ is left as the last accepted stress (the failed call's guess)
         if (guess_is_stress_) std::swap(guess_, sig_);
@@ -467,9 +472,13 @@ class Solver final : public SolverBase {
 
   void commit(double scale) override {
     for (auto& m : models_) m->commit(rot_, s_);
-    copy6(sig_old_, sig_);
-    copy6(eps_old_, eps_);
-    MTL_HIP(hipMemcpyAsync(D_old_, D_, 21 * n_ * sizeof(K), hipMemcpyDeviceToDevice, s_));
+    // the current stress, strain and tangent become the committed ones by
+    // swapping buffers; until the next step rewrites them the current values
+    // are read from the committed buffers (cur_*)
+    std::swap(sig_old_, sig_);
+    std::swap(eps_old_, eps_);
+    std::swap(D_old_, D_);
+    sig_stale_ = eps_stale_ = d_stale_ = true;
     scale6(fluct_, scale);
     stream_sync(s_);
   }
@@ -481,13 +490,13 @@ class Solver final : public SolverBase {
 
   void get_field(int which, double* out, int* ncomp) override {
     if (which == kFieldStress || which == kFieldStrain) {
-      soa_to_host(which == kFieldStress ? sig_ : eps_, 6, out);
+      soa_to_host(which == kFieldStress ? cur_sig() : cur_eps(), 6, out);
       *ncomp = 6;
       return;
     }
     if (which == kFieldTangent) {
       std::vector<K> soa(21 * size_t(n_));
-      MTL_HIP(hipMemcpy(soa.data(), D_, soa.size() * sizeof(K), hipMemcpyDeviceToHost));
+      MTL_HIP(hipMemcpy(soa.data(), cur_D(), soa.size() * sizeof(K), hipMemcpyDeviceToHost));
       for (long long p = 0; p < n_; ++p)
         for (int a = 0; a < 6; ++a)
           for (int b = 0; b < 6; ++b) out[p * 36 + a * 6 + b] = double(soa[size_t(sym6(a, b)) * n_ + p]);
@@ -528,7 +537,7 @@ class Solver final : public SolverBase {
     Vec6 mask;
     std::memcpy(mask.v, m, sizeof(mask.v));
     int istop = 0;
-    const int itn = minres_->solve(b, D_, mask, rtol, 5LL * 6LL * n_, s_, &istop);
+    const int itn = minres_->solve(b, cur_D(), mask, rtol, 5LL * 6LL * n_, s_, &istop);
     hipLaunchKernelGGL((k_convert<K, double>), dim3(grid_for(6 * n_)), dim3(kNBT), 0, s_, 6 * n_,
                        minres_->x(), x);
     MTL_LAUNCH_CHECK();
@@ -573,6 +582,12 @@ class Solver final : public SolverBase {
   // guess_ holds the stress of the last accepted constitutive call (the
   // buffers were swapped instead of copied) rather than the predictor's guess
   bool guess_is_stress_ = false;
+  // after a commit the current stress, strain and tangent live in the
+  // committed buffers until the next step (or constitutive call) rewrites them
+  bool sig_stale_ = false, eps_stale_ = false, d_stale_ = false;
+  const double* cur_sig() const { return sig_stale_ ? sig_old_ : sig_; }
+  const double* cur_eps() const { return eps_stale_ ? eps_old_ : eps_; }
+  K* cur_D() const { return d_stale_ ? D_old_ : D_; }
   double* partials_ = nullptr;
   double* hpart_ = nullptr;
   std::vector<ModelDesc> descs_;
