@@ -70,12 +70,15 @@ __global__ void __launch_bounds__(kNBT) k_guess(long long n, int step0, const do
 // eps += x + shift_c, fluct += x + shift_c (shift is per component)
 template <typename K>
 __global__ void __launch_bounds__(kNBT) k_add_correction(long long n, const K* x, double* eps,
-                                                          double* fluct, Vec6 shift) {
+                                                          double* fluct, const double* shift_dev) {
+  double shift[6];
+#pragma unroll
+  for (int c = 0; c < 6; ++c) shift[c] = shift_dev != nullptr ? shift_dev[c] : 0.0;
   for (long long p = blockIdx.x * (long long)kNBT + threadIdx.x; p < n;
        p += (long long)gridDim.x * kNBT) {
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-      const double d = double(x[c * n + p]) + shift.v[c];
+      const double d = double(x[c * n + p]) + shift[c];
       eps[c * n + p] += d;
       fluct[c * n + p] += d;
     }
@@ -95,6 +98,19 @@ __global__ void __launch_bounds__(kNBT) k_sum6(long long n, const K* x, double* 
   for (int c = 0; c < 6; ++c) {
     const double t = block_sum<kNBT>(acc[c], red);
     if (threadIdx.x == 0) partials[blockIdx.x * 6 + c] = t;
+  }
+}
+
+// shift[c] = -(sum of component c) / n for the strain-controlled components
+// (mask 0), 0 otherwise, from k_sum6's per-block partials (one workgroup)
+__global__ void __launch_bounds__(kNBT) k_mean_shift(int nb, const double* partials, long long n,
+                                                     Vec6 mask, double* shift) {
+  __shared__ double red[kNBT / 64];
+  for (int c = 0; c < 6; ++c) {
+    double acc = 0.0;
+    for (int i = threadIdx.x; i < nb; i += kNBT) acc += partials[i * 6 + c];
+    const double sum = block_sum<kNBT>(acc, red);
+    if (threadIdx.x == 0) shift[c] = mask.v[c] == 0.0 ? -sum / double(n) : 0.0;
   }
 }
 
@@ -228,11 +244,12 @@ class Solver final : public SolverBase {
     MTL_HIP(hipMalloc(&D_, 21 * n_ * sizeof(K)));
     MTL_HIP(hipMalloc(&D_old_, 21 * n_ * sizeof(K)));
     dalloc(partials_, kNormBlocks * 21);
+    dalloc(shift_, 6);
     MTL_HIP(hipHostMalloc(&hpart_, kNormBlocks * 21 * sizeof(double), hipHostMallocDefault));
   }
   ~Solver() override {
     models_.clear();
-    for (double* p : {eps_, sig_, sig_old_, eps_old_, fluct_, guess_, b_, rot_, partials_})
+    for (double* p : {eps_, sig_, sig_old_, eps_old_, fluct_, guess_, b_, rot_, partials_, shift_})
       if (p) (void)hipFree(p);
     if (D_) (void)hipFree(D_);
     if (D_old_) (void)hipFree(D_old_);
@@ -409,23 +426,19 @@ class Solver final : public SolverBase {
       if (debug_enabled())
         std::fprintf(stderr, "[mtl] step %d newton %d: minres %d iterations (istop %d)\n",
                      P.time_step_id, iteration, itn, istop);
-      Vec6 shift{};
       if (kMixed) {
         // The exact correction is compatible, with zero mean in every
         // strain-controlled component (Ghat[0] = diag(mask)); the FP32 FFTs
-        // leave a ~1e-7 relative mean there, removed here in FP64.
+        // leave a ~1e-7 relative mean there, removed here in FP64 (on the
+        // device: no host round trip).
         hipLaunchKernelGGL(k_sum6<K>, dim3(nb), dim3(kNBT), 0, s_, n_, minres_->x(), partials_);
         MTL_LAUNCH_CHECK();
-        MTL_HIP(hipMemcpyAsync(hpart_, partials_, nb * 6 * sizeof(double), hipMemcpyDeviceToHost, s_));
-        stream_sync(s_);
-        for (int c = 0; c < 6; ++c) {
-          double sum = 0.0;
-          for (int i = 0; i < nb; ++i) sum += hpart_[i * 6 + c];
-          shift.v[c] = mask.v[c] == 0.0 ? -sum / double(n_) : 0.0;
-        }
+        hipLaunchKernelGGL(k_mean_shift, dim3(1), dim3(kNBT), 0, s_, nb, partials_, n_, mask,
+                           shift_);
+        MTL_LAUNCH_CHECK();
       }
       hipLaunchKernelGGL(k_add_correction<K>, dim3(nb), dim3(kNBT), 0, s_, n_, minres_->x(), eps_,
-                         fluct_, shift);
+                         fluct_, kMixed ? shift_ : static_cast<const double*>(nullptr));
       MTL_LAUNCH_CHECK();
       const int it = constitutive_all(eps_, guess_, dt, sig_);
       sig_stale_ = false;
@@ -589,6 +602,7 @@ class Solver final : public SolverBase {
   const double* cur_eps() const { return eps_stale_ ? eps_old_ : eps_; }
   K* cur_D() const { return d_stale_ ? D_old_ : D_; }
   double* partials_ = nullptr;
+  double* shift_ = nullptr;  // mean shift of the mixed-form correction (6)
   double* hpart_ = nullptr;
   std::vector<ModelDesc> descs_;
   std::vector<std::unique_ptr<Model<K>>> models_;
