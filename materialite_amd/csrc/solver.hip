@@ -26,7 +26,7 @@ bool inverse6_host(const double* A, double* out);
 namespace {
 
 constexpr int kNBT = 256;
-constexpr int kNormBlocks = 1024;
+constexpr int kNormBlocks = 4096;
 constexpr int kNormWidth = 9;  // max|b|, sum sigma (6), max|dev dsigma|^2, max|D:dx|^2
 
 __global__ void __launch_bounds__(kNBT) k_predictor(long long n, const double* eps_old,
