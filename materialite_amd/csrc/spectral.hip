@@ -56,6 +56,7 @@ __host__ __device__ inline int zbuf_elems(int P, int nz, int nzc) {
 
 enum { kModeProject = 0, kModeMatvec = 1 };
 
+
 template <typename T>
 __device__ __forceinline__ void load_twiddles(cplx<T>* dst, const cplx<T>* src, int L) {
   for (int i = threadIdx.x; i < L; i += blockDim.x) dst[i] = src[i];
@@ -141,15 +142,35 @@ __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g,
   const int ny = g.ny;
   cplx<T>* base = spec + (long long)cx * ny * g.nzc + kz0;  // spec[c][x][.][kz0]
   load_twiddles(tw, twy, ny);
-  for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
-    const int y = i / TY, t = i - y * TY;
-    buf[y * TY + t] = t < width ? base[(long long)y * g.nzc + t] : cplx<T>{T(0), T(0)};
+  // FP32 tiles of whole, even-aligned columns move two complex values (16 B)
+  // per access (even ny keeps the LDS tile after the ny twiddles 16 B aligned)
+  const bool pairs = sizeof(T) == 4 && TY % 2 == 0 && width == TY && g.nzc % 2 == 0 && ny % 2 == 0;
+  const int hp = TY / 2;
+  if (pairs) {
+    for (int i = threadIdx.x; i < hp * ny; i += kYBT) {
+      const int y = i / hp, t = 2 * (i - y * hp);
+      *reinterpret_cast<float4*>(buf + y * TY + t) =
+          *reinterpret_cast<const float4*>(base + (long long)y * g.nzc + t);
+    }
+  } else {
+    for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
+      const int y = i / TY, t = i - y * TY;
+      buf[y * TY + t] = t < width ? base[(long long)y * g.nzc + t] : cplx<T>{T(0), T(0)};
+    }
   }
   __syncthreads();
   const cplx<T>* res = block_fft<T, kYBT>(buf, buf + TY * ny, TY, TY, plan, tw, inv);
-  for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
-    const int y = i / TY, t = i - y * TY;
-    if (t < width) base[(long long)y * g.nzc + t] = res[y * TY + t];
+  if (pairs) {
+    for (int i = threadIdx.x; i < hp * ny; i += kYBT) {
+      const int y = i / hp, t = 2 * (i - y * hp);
+      *reinterpret_cast<float4*>(base + (long long)y * g.nzc + t) =
+          *reinterpret_cast<const float4*>(res + y * TY + t);
+    }
+  } else {
+    for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
+      const int y = i / TY, t = i - y * TY;
+      if (t < width) base[(long long)y * g.nzc + t] = res[y * TY + t];
+    }
   }
 }
 
@@ -210,12 +231,26 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   cplx<T>* base = spec + (long long)y * g.nzc + kz0;  // spec[c][0][y][kz0]
   load_twiddles(tw, twx, nx);
   const int tot = 6 * nx * TX;
-  for (int i = threadIdx.x; i < tot; i += kXBT) {
-    const int t = i % TX;
-    const int cxp = i / TX;  // c*nx + x
-    const int c = cxp / nx, x = cxp - c * nx;
-    buf[x * (6 * TX) + c * TX + t] =
-        t < width ? base[c * g.nspec + x * xstride + t] : cplx<T>{T(0), T(0)};
+  // FP32 tiles of whole, even-aligned columns move two complex values (16 B)
+  // per access (even nx keeps the LDS tile after the nx twiddles 16 B aligned)
+  const bool pairs = sizeof(T) == 4 && TX % 2 == 0 && width == TX && g.nzc % 2 == 0 && nx % 2 == 0;
+  const int hp = TX / 2;
+  if (pairs) {
+    for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
+      const int cxp = i / hp;  // c*nx + x
+      const int t = 2 * (i - cxp * hp);
+      const int c = cxp / nx, x = cxp - c * nx;
+      *reinterpret_cast<float4*>(buf + x * (6 * TX) + c * TX + t) =
+          *reinterpret_cast<const float4*>(base + c * g.nspec + x * xstride + t);
+    }
+  } else {
+    for (int i = threadIdx.x; i < tot; i += kXBT) {
+      const int t = i % TX;
+      const int cxp = i / TX;  // c*nx + x
+      const int c = cxp / nx, x = cxp - c * nx;
+      buf[x * (6 * TX) + c * TX + t] =
+          t < width ? base[c * g.nspec + x * xstride + t] : cplx<T>{T(0), T(0)};
+    }
   }
   __syncthreads();
   const int nyqx = (nx % 2 == 0) ? nx / 2 : -1;
@@ -248,11 +283,21 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   __syncthreads();
   const cplx<T>* out =
       block_fft<T, kXBT>(res, res == buf ? buf2 : buf, 6 * TX, 6 * TX, plan, tw, true);
-  for (int i = threadIdx.x; i < tot; i += kXBT) {
-    const int t = i % TX;
-    const int cxp = i / TX;
-    const int c = cxp / nx, x = cxp - c * nx;
-    if (t < width) base[c * g.nspec + x * xstride + t] = out[x * (6 * TX) + c * TX + t];
+  if (pairs) {
+    for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
+      const int cxp = i / hp;
+      const int t = 2 * (i - cxp * hp);
+      const int c = cxp / nx, x = cxp - c * nx;
+      *reinterpret_cast<float4*>(base + c * g.nspec + x * xstride + t) =
+          *reinterpret_cast<const float4*>(out + x * (6 * TX) + c * TX + t);
+    }
+  } else {
+    for (int i = threadIdx.x; i < tot; i += kXBT) {
+      const int t = i % TX;
+      const int cxp = i / TX;
+      const int c = cxp / nx, x = cxp - c * nx;
+      if (t < width) base[c * g.nspec + x * xstride + t] = out[x * (6 * TX) + c * TX + t];
+    }
   }
 }
 
