@@ -14,6 +14,7 @@
 #include <algorithm>
 #include <cmath>
 #include <mutex>
+#include <type_traits>
 #include <unordered_map>
 #include <vector>
 
@@ -310,17 +311,20 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
 // transformed, the registers already hold the loads of component c + 1.
 // Keeping one component in LDS (instead of all six) keeps tiles 128 B wide and
 // four workgroups per CU.
-template <typename T>
+// With PAIRS (FP32, even TX dividing nzc, even nx) a thread moves two
+// adjacent complex values (16 B) per access and the LDS rows have stride TX.
+template <typename T, bool PAIRS>
 __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
                                                   const T* tanz, int maskbits,
                                                   const MinresState* st, cplx<T>* spec) {
+  static_assert(!PAIRS || sizeof(T) == 4, "pairs of complex values are 16 B in FP32 only");
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
   const int nx = g.nx;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int S = odd_stride(TX);
+  const int S = PAIRS ? TX : odd_stride(TX);
   cplx<T>* buf = tw + nx;
   cplx<T>* buf2 = buf + nx * S;
   const int b = xcd_logical_block(blockIdx.x, gridDim.x);
@@ -331,38 +335,42 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan
   const long long xstride = (long long)g.ny * g.nzc;
   cplx<T>* base = spec + (long long)y * g.nzc + kz0;  // spec[c][0][y][kz0]
   load_twiddles(tw, twx, nx);
-  // per-thread elements of one component tile: (x, t) with t fastest
-  constexpr int K = kXTileK;
-  const int E = nx * TX;
+  // per-thread units of one component tile (a complex value, or a pair of
+  // adjacent ones with PAIRS): (x, t) with t fastest
+  constexpr int W = PAIRS ? 2 : 1;        // complex values per unit
+  constexpr int K = kXTileK / W;          // units per thread
+  using Unit = typename std::conditional<PAIRS, float4, cplx<T>>::type;
+  const int TU = TX / W;                  // units per row
+  const int E = nx * TU;
   int goff[K], loff[K];
   bool on[K];
 #pragma unroll
   for (int k = 0; k < K; ++k) {
     const int i = threadIdx.x + k * kXBT;
-    const int x = i / TX, t = i - x * TX;
+    const int x = i / TU, t = W * (i - x * TU);
     goff[k] = x * int(xstride) + t;
     loff[k] = x * S + t;
     on[k] = i < E && t < width;
   }
-  cplx<T> pre[K];
+  Unit pre[K];
   auto fetch = [&](int c) {
-    const cplx<T>* src = base + c * g.nspec;
+    const Unit* src = reinterpret_cast<const Unit*>(base + c * g.nspec);
 #pragma unroll
-    for (int k = 0; k < K; ++k) pre[k] = on[k] ? src[goff[k]] : cplx<T>{T(0), T(0)};
+    for (int k = 0; k < K; ++k) pre[k] = on[k] ? src[goff[k] / W] : Unit{};
   };
   auto transform = [&](bool inv) {
     fetch(0);
     for (int c = 0; c < 6; ++c) {
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        if (threadIdx.x + k * kXBT < E) buf[loff[k]] = pre[k];
+        if (threadIdx.x + k * kXBT < E) reinterpret_cast<Unit*>(buf)[loff[k] / W] = pre[k];
       lds_barrier();
       if (c < 5) fetch(c + 1);
       const cplx<T>* res = block_fft<T, kXBT>(buf, buf2, TX, S, plan, tw, inv);
-      cplx<T>* dst = base + c * g.nspec;
+      Unit* dst = reinterpret_cast<Unit*>(base + c * g.nspec);
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        if (on[k]) dst[goff[k]] = res[loff[k]];
+        if (on[k]) dst[goff[k] / W] = reinterpret_cast<const Unit*>(res)[loff[k] / W];
       lds_barrier();
     }
   };
@@ -579,6 +587,8 @@ Spectral<T>::Spectral(int nx, int ny, int nz) {
       tx_ = 1;
     if (tx_ == 0) throw Error("nx too large for the x pass");
     tx_ = std::min(tx_, g_.nzc);
+    // FP32 with whole, even-width tiles: pairs of complex values per access
+    x_pairs_ = sizeof(T) == 4 && tx_ % 2 == 0 && g_.nzc % tx_ == 0 && nx % 2 == 0;
   }
   MTL_HIP(hipMalloc(&spec_, 6 * g_.nspec * sizeof(cplx<T>)));
   twx_ = upload(twiddle_table<T>(nx));
@@ -626,9 +636,17 @@ void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const V
   MTL_LAUNCH_CHECK();
 
   const int xtiles = ceil_div(g_.nzc, tx_);
-  const size_t xlds = x_resident_ ? (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<T>)
-                                  : x_lds_bytes(g_.nx, tx_, sizeof(cplx<T>));
-  auto kx = x_resident_ ? k_x_ghat_resident<T> : k_x_ghat_streamed<T>;
+  const size_t xlds =
+      x_resident_ ? (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<T>)
+      : x_pairs_  ? (size_t(g_.nx) + 2 * size_t(tx_) * g_.nx) * sizeof(cplx<T>)
+                  : x_lds_bytes(g_.nx, tx_, sizeof(cplx<T>));
+  auto kx = k_x_ghat_resident<T>;
+  if (!x_resident_) {
+    if constexpr (sizeof(T) == 4)
+      kx = x_pairs_ ? k_x_ghat_streamed<T, true> : k_x_ghat_streamed<T, false>;
+    else
+      kx = k_x_ghat_streamed<T, false>;
+  }
   allow_lds(kx, xlds);
   profiler().launch(MODE == kModeMatvec ? kKXGhat : residual_id(kKXGhat), kx, dim3(g_.ny * xtiles), dim3(kXBT), xlds, s, g_, pxd_, twx_, tx_, xtiles, tanx_, tany_, tanz_, mask_bits(mask), stp, spec_);
   MTL_LAUNCH_CHECK();

@@ -52,6 +52,7 @@ class Spectral {
   int ty_;        // kz columns per workgroup in the y pass
   int tx_;        // kz columns per workgroup in the x pass
   bool x_resident_;  // x pass keeps all six components of its tile in LDS
+  bool x_pairs_ = false;  // streamed x pass moves pairs of complex values
   cplx<T>* spec_ = nullptr;
   cplx<T>* twx_ = nullptr;
   cplx<T>* twy_ = nullptr;

@@ -26,10 +26,11 @@ def _need_gpu():
 # The long-x grids (nx >= 250) take the x pass that streams the components
 # through LDS one at a time (six resident components do not fit two kz
 # columns there), with full, partial (256 x 2 x 40: nzc = 20 in tiles of 8) and
-# odd-width (256 x 4 x 6: nzc = 3) tiles and a radix-5 length (250).
+# odd-width (256 x 4 x 6: nzc = 3) tiles and a radix-5 length (250); in FP32,
+# 250 x 3 x 4 and 256 x 3 x 32 (whole even-width tiles) move pairs of values.
 GRIDS = [(3, 4, 5), (4, 4, 4), (2, 3, 6), (1, 4, 3), (6, 5, 8), (8, 8, 8), (7, 9, 10),
          (16, 16, 16), (12, 10, 14), (32, 32, 32), (48, 48, 48),
-         (256, 4, 6), (256, 2, 40), (250, 3, 4), (512, 2, 6)]
+         (256, 4, 6), (256, 2, 40), (250, 3, 4), (512, 2, 6), (256, 3, 32)]
 
 
 @pytest.mark.parametrize("dims", GRIDS)
