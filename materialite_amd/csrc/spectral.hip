@@ -428,9 +428,21 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan
   cplx<T>* stage = buf + zb;
   const int run = P * nzc;
   const long long valid = (nlines - line0 < P ? nlines - line0 : P) * nzc;
-  for (int i = threadIdx.x; i < 6 * run; i += kZBT) {
-    const int c = i / run, r = i - c * run;
-    stage[i] = r < valid ? spec[c * g.nspec + line0 * nzc + r] : cplx<T>{T(0), T(0)};
+  if (sizeof(T) == 4 && nzc % 2 == 0 && nz % 2 == 0) {
+    // FP32, even nzc: two adjacent values per access (16 B; even nz keeps the
+    // LDS stage 16 B aligned)
+    const int hr = run / 2;
+    for (int i = threadIdx.x; i < 6 * hr; i += kZBT) {
+      const int c = i / hr, r = 2 * (i - c * hr);
+      const float4 v = r < valid ? *reinterpret_cast<const float4*>(spec + c * g.nspec + line0 * nzc + r)
+                                 : make_float4(0.f, 0.f, 0.f, 0.f);
+      *reinterpret_cast<float4*>(stage + c * run + r) = v;
+    }
+  } else {
+    for (int i = threadIdx.x; i < 6 * run; i += kZBT) {
+      const int c = i / run, r = i - c * run;
+      stage[i] = r < valid ? spec[c * g.nspec + line0 * nzc + r] : cplx<T>{T(0), T(0)};
+    }
   }
   __syncthreads();
   for (int i = threadIdx.x; i < P * 3 * nz; i += kZBT) {
