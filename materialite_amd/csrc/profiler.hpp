@@ -1,7 +1,7 @@
 // Optional per-kernel timing.  Kernels launched through Profiler::launch carry
 // a start and a stop event on their own dispatch (hipExtLaunchKernelGGL), so a
 // figure is the kernel's execution time, with no marker packets between
-// kernels; begin()/end() bracket other stream work with event records.
+// kernels.
 //
 // After a host synchronisation the device has idled while the host ran, and
 // the start event of the next dispatch would also cover its wake-up (tens of
@@ -70,18 +70,6 @@ class Profiler {
   // the next launch follows a host synchronisation
   void after_sync() { wake_next_ = enabled; }
 
-  void begin(hipStream_t s) {
-    if (!enabled) return;
-    cur_ = take();
-    MTL_HIP(hipEventRecord(cur_, s));
-  }
-  void end(int id, hipStream_t s) {
-    if (!enabled) return;
-    hipEvent_t b = take();
-    MTL_HIP(hipEventRecord(b, s));
-    pending_.push_back({id, cur_, b});
-    if (pending_.size() > 4096) flush();
-  }
   void flush() {
     for (auto& r : pending_) {
       MTL_HIP(hipEventSynchronize(r.b));
@@ -121,7 +109,6 @@ class Profiler {
     return e;
   }
   bool wake_next_ = false;
-  hipEvent_t cur_ = nullptr;
   std::vector<Rec> pending_;
   std::vector<hipEvent_t> pool_;
 };
