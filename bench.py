@@ -64,7 +64,10 @@ def parse():
                     help="print the per-dispatch HBM traffic summary of two rocprofv3 --pmc runs")
     ap.add_argument("--summarize-trace", metavar="KERNEL_TRACE_CSV",
                     help="print the per-kernel table of a rocprofv3 --kernel-trace run")
+    ap.add_argument("--config1", type=int, default=1,
+                    help="time the 16^3 elastic workload on the GPU and with the oracle on the CPU")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--config1-child", metavar="NPY", help=argparse.SUPPRESS)
     ap.add_argument("--cpu-iterations", type=int, default=4, help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -431,6 +434,86 @@ def cpu_baseline_child(n, grains, iters):
     print(json.dumps({"vi_per_s": npts * itn / dt, "iterations": itn, "seconds": dt}))
 
 
+def host_info():
+    """CPU model, logical CPUs and NumPy/SciPy versions of this host."""
+    import numpy as np
+    import scipy
+
+    model = ""
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return {"cpu_model": model, "logical_cpus": os.cpu_count(), "numpy": np.__version__,
+            "scipy": scipy.__version__}
+
+
+def config1_child(npy_path):
+    """The oracle's full config-1 run (16^3, 20 grains, elastic cubic, uniaxial x,
+    10 steps of dt 1, tolerances 1e-8 / 1e-10) on one thread: seconds and the
+    stress field."""
+    os.environ.setdefault("OMP_NUM_THREADS", "1")
+    import numpy as np
+
+    from oracle import constitutive as C
+    from oracle import tensors as T
+    from oracle.solver import LoadSchedule as OLS
+    from oracle.solver import SmallStrainFFT as OSolver
+
+    grain_of_point, rot = workload(16, 20, seed=20260001)
+    model = C.Elastic(T.cubic_stiffness(250e3, 150e3, 120e3))
+    solver = OSolver(OLS.from_constant_uniaxial_strain_rate(1e-3, "x"), end_time=10.0,
+                     initial_time_increment=1.0)
+    t0 = time.perf_counter()
+    out = solver.run([16, 16, 16], rot[grain_of_point], model, global_tolerance=1e-8,
+                     linear_solver_tolerance=1e-10)
+    dt = time.perf_counter() - t0
+    np.save(npy_path, out["stress"])
+    print(json.dumps({"seconds": dt, "krylov": int(solver.stats["krylov_iterations"])}))
+
+
+def config1():
+    """Config 1 end to end: the GPU solve (both forms, whole run) against the
+    oracle's single-thread run of the same input."""
+    import numpy as np
+
+    from materialite_amd import serving
+    from materialite_amd.models.small_strain_fft import LoadSchedule, SmallStrainFFT
+
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        env[k] = "1"
+    import tempfile
+
+    with tempfile.TemporaryDirectory() as tmp:
+        npy = os.path.join(tmp, "oracle_stress.npy")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--config1-child", npy],
+                           capture_output=True, text=True, env=env, timeout=600)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr[-2000:])
+        cpu = json.loads(r.stdout.strip().splitlines()[-1])
+        ref = np.load(npy)
+    material = build_material(16, 20, seed=20260001)
+    res = {"workload": "16^3 elastic cubic, 20 grains (seed 20260001), uniaxial x 1e-3/s, 10 steps "
+                       "of dt 1, tolerances 1e-8 / 1e-10, whole run",
+           "cpu_oracle_seconds": cpu["seconds"], "cpu_oracle_krylov": cpu["krylov"],
+           "cpu_threads": 1}
+    for prec in ("fp64", "mixed"):
+        solver = SmallStrainFFT(LoadSchedule.from_constant_uniaxial_strain_rate(1e-3, "x"),
+                                end_time=10.0, initial_time_increment=1.0,
+                                constitutive_model=serving.cubic_elastic(), precision=prec)
+        solver.run(material, global_tolerance=1e-8, linear_solver_tolerance=1e-10)  # warm
+        t0 = time.perf_counter()
+        out = solver.run(material, global_tolerance=1e-8, linear_solver_tolerance=1e-10)
+        res[f"gpu_seconds_{prec}"] = time.perf_counter() - t0
+        s = out.extract("stress").components
+        res[f"max_rel_stress_diff_{prec}_vs_oracle"] = float(np.abs(s - ref).max() / np.abs(ref).max())
+    return res
+
+
 def cpu_baseline(n, grains, iters):
     env = dict(os.environ)
     for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
@@ -445,6 +528,7 @@ def cpu_baseline(n, grains, iters):
         "value": res["vi_per_s"],
         "unit": "voxel-iters/s",
         "cores": 1,
+        "host": host_info(),
         "kind": "port",
         "sample": (f"oracle FP64 NumPy MINRES, {res['iterations']} Krylov iterations of the first "
                    f"Newton correction of the {n}^3 EVP workload ({res['seconds']:.1f} s, 1 thread)"),
@@ -455,6 +539,9 @@ def main():
     args = parse()
     if args.cpu_baseline_child:
         cpu_baseline_child(args.grid, args.grains, args.cpu_iterations)
+        return
+    if args.config1_child:
+        config1_child(args.config1_child)
         return
     if args.summarize_trace:
         print_trace_summary(args.summarize_trace)
@@ -511,6 +598,7 @@ def main():
     study = precision_study() if args.precision_study else None
     large = large_two_phase() if args.large else None
     cpu = cpu_baseline(n, args.grains, args.cpu_iterations) if args.cpu_baseline else None
+    c1 = config1() if args.config1 else None
     line = {
         "metric": "voxel-iters/s at 128³ EVP, FP32-refined vs FP64; p50/p99 solve latency mixed",
         "value": vi,
@@ -551,6 +639,7 @@ def main():
         "two_phase_256": large,
         "vi_roofline_frac": vi * bytes_per_vi(n, args.precision) / (HBM_PEAK_GBS * 1e9),
         "cpu_baseline": cpu,
+        "config1_16": c1,
     }
     print(json.dumps(line))
 
