@@ -67,7 +67,7 @@ def parse():
     ap.add_argument("--config1", type=int, default=1,
                     help="time the 16^3 elastic workload on the GPU and with the oracle on the CPU")
     ap.add_argument("--cpu-baseline-child", action="store_true", help=argparse.SUPPRESS)
-    ap.add_argument("--config1-child", metavar="NPY", help=argparse.SUPPRESS)
+    ap.add_argument("--oracle-child", nargs=2, metavar=("SPEC", "NPY"), help=argparse.SUPPRESS)
     ap.add_argument("--cpu-iterations", type=int, default=4, help=argparse.SUPPRESS)
     return ap.parse_args()
 
@@ -176,6 +176,13 @@ def precision_study(n=64, grains=200, steps=10):
                                    global_tolerance=1e-8, linear_solver_tolerance=1e-10)
         (sm, km, tm), (sd, kd, td) = res["mixed"], res["fp64"]
         vim, vid = n ** 3 * km / tm, n ** 3 * kd / td
+        print(f"[bench] precision study: {name} oracle, first step", file=sys.stderr, flush=True)
+        cpu, _ = run_oracle({"n": n, "grains": grains, "seed": 20260002,
+                             "load": "x" if name == "uniaxial_x" else "xy", "end_time": float(steps),
+                             "dt": 1.0, "tol": 1e-8, "lin_tol": 1e-10, "max_steps": 1})
+        out[name + "_cpu_oracle_first_step"] = {
+            "seconds": cpu["seconds"], "krylov": cpu["krylov"], "threads": 1,
+            "vi_per_s": n ** 3 * cpu["krylov"] / cpu["seconds"]}
         out[name] = {
             "max_rel_stress_error": float(np.abs(sm - sd).max() / np.abs(sd).max()),
             "krylov_mixed": int(km), "krylov_fp64": int(kd),
@@ -451,10 +458,10 @@ def host_info():
             "scipy": scipy.__version__}
 
 
-def config1_child(npy_path):
-    """The oracle's full config-1 run (16^3, 20 grains, elastic cubic, uniaxial x,
-    10 steps of dt 1, tolerances 1e-8 / 1e-10) on one thread: seconds and the
-    stress field."""
+def oracle_child(spec_json, npy_path):
+    """One oracle run on one thread (spec: grid n, grains, seed, load "x" or "xy",
+    end_time, dt, tolerances, max_steps): prints seconds and Krylov iterations,
+    saves the stress field."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     import numpy as np
 
@@ -463,16 +470,46 @@ def config1_child(npy_path):
     from oracle.solver import LoadSchedule as OLS
     from oracle.solver import SmallStrainFFT as OSolver
 
-    grain_of_point, rot = workload(16, 20, seed=20260001)
+    sp = json.loads(spec_json)
+    n = sp["n"]
+    grain_of_point, rot = workload(n, sp["grains"], seed=sp["seed"])
     model = C.Elastic(T.cubic_stiffness(250e3, 150e3, 120e3))
-    solver = OSolver(OLS.from_constant_uniaxial_strain_rate(1e-3, "x"), end_time=10.0,
-                     initial_time_increment=1.0)
+    if sp["load"] == "x":
+        ls = OLS.from_constant_uniaxial_strain_rate(1e-3, "x")
+    else:  # pure xy shear: Mandel slot 5 at sqrt(2) 1e-3, the other stresses zero
+        rate = np.zeros(6)
+        rate[5] = np.sqrt(2.0) * 1e-3
+        mask = np.ones(6)
+        mask[5] = 0.0
+        ls = OLS.from_constant_rates(rate, np.zeros(6), mask)
+    solver = OSolver(ls, end_time=sp["end_time"], initial_time_increment=sp["dt"],
+                     max_time_increment=sp["dt"])
     t0 = time.perf_counter()
-    out = solver.run([16, 16, 16], rot[grain_of_point], model, global_tolerance=1e-8,
-                     linear_solver_tolerance=1e-10)
+    out = solver.run([n, n, n], rot[grain_of_point], model, global_tolerance=sp["tol"],
+                     linear_solver_tolerance=sp["lin_tol"], max_steps=sp.get("max_steps"))
     dt = time.perf_counter() - t0
     np.save(npy_path, out["stress"])
-    print(json.dumps({"seconds": dt, "krylov": int(solver.stats["krylov_iterations"])}))
+    print(json.dumps({"seconds": dt, "krylov": int(solver.stats["krylov_iterations"]),
+                      "steps": int(solver.stats["steps"])}))
+
+
+def run_oracle(spec):
+    """oracle_child in a one-thread subprocess -> (result dict, stress field)."""
+    import tempfile
+
+    import numpy as np
+
+    env = dict(os.environ)
+    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
+        env[k] = "1"
+    with tempfile.TemporaryDirectory() as tmp:
+        npy = os.path.join(tmp, "oracle_stress.npy")
+        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--oracle-child",
+                            json.dumps(spec), npy], capture_output=True, text=True, env=env,
+                           timeout=900)
+        if r.returncode != 0:
+            raise RuntimeError(r.stderr[-2000:])
+        return json.loads(r.stdout.strip().splitlines()[-1]), np.load(npy)
 
 
 def config1():
@@ -483,19 +520,8 @@ def config1():
     from materialite_amd import serving
     from materialite_amd.models.small_strain_fft import LoadSchedule, SmallStrainFFT
 
-    env = dict(os.environ)
-    for k in ("OMP_NUM_THREADS", "OPENBLAS_NUM_THREADS", "MKL_NUM_THREADS"):
-        env[k] = "1"
-    import tempfile
-
-    with tempfile.TemporaryDirectory() as tmp:
-        npy = os.path.join(tmp, "oracle_stress.npy")
-        r = subprocess.run([sys.executable, os.path.abspath(__file__), "--config1-child", npy],
-                           capture_output=True, text=True, env=env, timeout=600)
-        if r.returncode != 0:
-            raise RuntimeError(r.stderr[-2000:])
-        cpu = json.loads(r.stdout.strip().splitlines()[-1])
-        ref = np.load(npy)
+    cpu, ref = run_oracle({"n": 16, "grains": 20, "seed": 20260001, "load": "x", "end_time": 10.0,
+                           "dt": 1.0, "tol": 1e-8, "lin_tol": 1e-10})
     material = build_material(16, 20, seed=20260001)
     res = {"workload": "16^3 elastic cubic, 20 grains (seed 20260001), uniaxial x 1e-3/s, 10 steps "
                        "of dt 1, tolerances 1e-8 / 1e-10, whole run",
@@ -540,8 +566,8 @@ def main():
     if args.cpu_baseline_child:
         cpu_baseline_child(args.grid, args.grains, args.cpu_iterations)
         return
-    if args.config1_child:
-        config1_child(args.config1_child)
+    if args.oracle_child:
+        oracle_child(*args.oracle_child)
         return
     if args.summarize_trace:
         print_trace_summary(args.summarize_trace)
