@@ -565,9 +565,6 @@ class Solver final : public SolverBase {
   }
 
  private:
-  void copy6(double* dst, const double* src) {
-    MTL_HIP(hipMemcpyAsync(dst, src, 6 * n_ * sizeof(double), hipMemcpyDeviceToDevice, s_));
-  }
   void scale6(double* a, double f) {
     hipLaunchKernelGGL(k_scale, dim3(grid_for(6 * n_)), dim3(kNBT), 0, s_, 6 * n_, a, f);
     MTL_LAUNCH_CHECK();
