@@ -203,6 +203,43 @@ struct Dft<T, 7> {
   }
 };
 
+// Odd prime R (11, 13): the symmetric form of the radix-7 DFT above written
+// for any odd R, with the roots w_m = exp(-2 pi i m / R) read from the table:
+//   y_u, y_{R-u} = v_0 + sum_k Re(w_uk) a_k  +- rot(sum_k -Im(w_uk) b_k)
+template <typename T, int R>
+struct DftOdd {
+  __device__ __forceinline__ static void run(cplx<T>* v, bool inv, const cplx<T>* rt, int q) {
+    constexpr int H = (R - 1) / 2;
+    cplx<T> a[H], b[H];
+#pragma unroll
+    for (int k = 1; k <= H; ++k) {
+      a[k - 1] = cadd(v[k], v[R - k]);
+      b[k - 1] = csub(v[k], v[R - k]);
+    }
+    const cplx<T> x0 = v[0];
+    cplx<T> y0 = x0;
+#pragma unroll
+    for (int k = 0; k < H; ++k) y0 = cadd(y0, a[k]);
+#pragma unroll
+    for (int u = 1; u <= H; ++u) {
+      cplx<T> m = x0, n = {T(0), T(0)};
+#pragma unroll
+      for (int k = 1; k <= H; ++k) {
+        const cplx<T> w = rt[((u * k) % R) * q];
+        m = {m.x + w.x * a[k - 1].x, m.y + w.x * a[k - 1].y};
+        n = {n.x - w.y * b[k - 1].x, n.y - w.y * b[k - 1].y};
+      }
+      const cplx<T> r = rot_i(n, inv);
+      v[u] = cadd(m, r);
+      v[R - u] = csub(m, r);
+    }
+    v[0] = y0;
+  }
+};
+template <typename T>
+struct Dft<T, 11> : DftOdd<T, 11> {};
+template <typename T>
+struct Dft<T, 13> : DftOdd<T, 13> {};
 
 // --------------------------------------------------------------- one butterfly
 template <typename T, int R>
@@ -232,7 +269,11 @@ __device__ __forceinline__ void butterfly(const cplx<T>* __restrict__ src, cplx<
 // the per-pass index arithmetic is shared by all radices.  Butterfly
 // bf = j * M + m of a thread advances by BT per step; (j, m) is stepped
 // incrementally instead of divided out.
-template <typename T, int BT>
+//
+// BIG adds the radix-11 and radix-13 butterflies; they are compiled only into
+// the kernels of axes whose plan uses them, since their registers would
+// otherwise lower the occupancy of every pass.
+template <typename T, int BT, bool BIG = false>
 __device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int S,
                                               const AxisPlan* __restrict__ pl, const cplx<T>* tw,
                                               bool inv) {
@@ -258,6 +299,12 @@ __device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int
         case 3: butterfly<T, 3>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
         case 5: butterfly<T, 5>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
         case 7: butterfly<T, 7>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv); break;
+        case 11:
+          if constexpr (BIG) butterfly<T, 11>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv);
+          break;
+        case 13:
+          if constexpr (BIG) butterfly<T, 13>(a, b, S, j, m, kk, jq, per_line, Ns, tstride, tw, inv);
+          break;
         default: break;  // rejected on the host when the plan is built
       }
       j += dj;
@@ -275,13 +322,14 @@ __device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int
   return a;
 }
 
-// Host: factor L into radices 16,8,4,2,3,5,7 (largest first). Throws if impossible.
+// Host: factor L into radices 16,8,4,2,3,5,7,11,13 (largest powers of two first).
+// Throws if impossible.  uses_big_radix(): the plan needs the BIG block_fft.
 inline AxisPlan make_axis_plan(int L) {
   AxisPlan p{};
   p.L = L;
   p.npass = 0;
   int rem = L, ns = 1;
-  const int order[7] = {16, 8, 4, 2, 3, 5, 7};
+  const int order[9] = {16, 8, 4, 2, 3, 5, 7, 11, 13};
   for (int r : order) {
     while (rem % r == 0 && rem > 1) {
       if (p.npass >= kMaxPasses) throw Error("FFT plan: too many passes");
@@ -295,9 +343,15 @@ inline AxisPlan make_axis_plan(int L) {
   }
   if (rem != 1)
     throw Error("grid dimension " + std::to_string(L) +
-                " has a prime factor above 7; the FFT supports radices 2, 3, 4, 5, 7, 8");
+                " has a prime factor above 13; the FFT supports radices 2-5, 7, 8, 11, 13, 16");
   if (L >= (1 << 16)) throw Error("grid dimension " + std::to_string(L) + " too large");
   return p;
+}
+
+inline bool uses_big_radix(const AxisPlan& p) {
+  for (int i = 0; i < p.npass; ++i)
+    if (p.radix[i] > 8 && p.radix[i] != 16) return true;
+  return false;
 }
 
 }  // namespace mtl

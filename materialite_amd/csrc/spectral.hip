@@ -64,7 +64,7 @@ __device__ __forceinline__ void load_twiddles(cplx<T>* dst, const cplx<T>* src, 
 }
 
 // ----------------------------------------------------------------- z forward
-template <typename T, int MODE, typename TIn>
+template <typename T, int MODE, typename TIn, bool BIG>
 __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan_g, const cplx<T>* twz,
                                                      int P, const TIn* in, const T* tangent,
                                                      Vec6 a, double sgn,
@@ -108,7 +108,7 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan
     for (int q = 0; q < 3; ++q) buf[z * S + li * 3 + q] = {tau[2 * q], tau[2 * q + 1]};
   }
   __syncthreads();
-  const cplx<T>* res = block_fft<T, kZBT>(buf, buf + zb, M, S, plan, tw, false);
+  const cplx<T>* res = block_fft<T, kZBT, BIG>(buf, buf + zb, M, S, plan, tw, false);
 
   const T half = T(0.5);
   for (int i = threadIdx.x; i < P * 3 * nzc; i += kZBT) {
@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan
 }
 
 // ----------------------------------------------------------------- y pass
-template <typename T>
+template <typename T, bool BIG>
 __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g, const cplx<T>* twy, int TY,
                                                   int ntiles, bool inv,
                                                   const MinresState* st, cplx<T>* spec) {
@@ -160,7 +160,7 @@ __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g,
     }
   }
   __syncthreads();
-  const cplx<T>* res = block_fft<T, kYBT>(buf, buf + TY * ny, TY, TY, plan, tw, inv);
+  const cplx<T>* res = block_fft<T, kYBT, BIG>(buf, buf + TY * ny, TY, TY, plan, tw, inv);
   if (pairs) {
     for (int i = threadIdx.x; i < hp * ny; i += kYBT) {
       const int y = i / hp, t = 2 * (i - y * hp);
@@ -212,7 +212,7 @@ __device__ __forceinline__ double rnorm3<double>(double a, double b, double c) {
 // LDS: forward x-FFT of the 6 TX interleaved lines, Ghat on each point, inverse
 // x-FFT, one HBM read and one write of the tile.  Used where TX >= 2 fits the
 // budget (grids up to about 128 points along x).
-template <typename T>
+template <typename T, bool BIG>
 __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
                                                   const T* tanz, int maskbits,
@@ -259,7 +259,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   const int nyqz = (g.nz % 2 == 0) ? g.nz / 2 : -1;  // never stored, kept for clarity
   const T ty = tany[y];
   cplx<T>* buf2 = buf + 6 * TX * nx;
-  cplx<T>* res = block_fft<T, kXBT>(buf, buf2, 6 * TX, 6 * TX, plan, tw, false);
+  cplx<T>* res = block_fft<T, kXBT, BIG>(buf, buf2, 6 * TX, 6 * TX, plan, tw, false);
   for (int i = threadIdx.x; i < nx * TX; i += kXBT) {
     const int t = i % TX, kx = i / TX;
     const int kz = kz0 + t;
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   }
   __syncthreads();
   const cplx<T>* out =
-      block_fft<T, kXBT>(res, res == buf ? buf2 : buf, 6 * TX, 6 * TX, plan, tw, true);
+      block_fft<T, kXBT, BIG>(res, res == buf ? buf2 : buf, 6 * TX, 6 * TX, plan, tw, true);
   if (pairs) {
     for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
       const int cxp = i / hp;
@@ -313,7 +313,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
 // four workgroups per CU.
 // With PAIRS (FP32, even TX dividing nzc, even nx) a thread moves two
 // adjacent complex values (16 B) per access and the LDS rows have stride TX.
-template <typename T, bool PAIRS>
+template <typename T, bool PAIRS, bool BIG>
 __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
                                                   const T* tanz, int maskbits,
@@ -366,7 +366,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan
         if (threadIdx.x + k * kXBT < E) reinterpret_cast<Unit*>(buf)[loff[k] / W] = pre[k];
       lds_barrier();
       if (c < 5) fetch(c + 1);
-      const cplx<T>* res = block_fft<T, kXBT>(buf, buf2, TX, S, plan, tw, inv);
+      const cplx<T>* res = block_fft<T, kXBT, BIG>(buf, buf2, TX, S, plan, tw, inv);
       Unit* dst = reinterpret_cast<Unit*>(base + c * g.nspec);
 #pragma unroll
       for (int k = 0; k < K; ++k)
@@ -407,7 +407,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan
 }
 
 // ----------------------------------------------------------------- z inverse
-template <typename T, int MODE, typename TOut>
+template <typename T, int MODE, typename TOut, bool BIG>
 __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan_g, const cplx<T>* twz,
                                                      int P, const cplx<T>* spec, TOut* out,
                                                      T* r1, const T* r2,
@@ -465,7 +465,7 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan
     buf[k * S + li * 3 + q] = zv;
   }
   __syncthreads();
-  const cplx<T>* res = block_fft<T, kZBT>(buf, buf + zb, M, S, plan, tw, true);
+  const cplx<T>* res = block_fft<T, kZBT, BIG>(buf, buf + zb, M, S, plan, tw, true);
 
   const T invn = T(1.0 / double(g.n));
   double acc = 0.0;
@@ -566,6 +566,9 @@ Spectral<T>::Spectral(int nx, int ny, int nz) {
   px_ = make_axis_plan(nx);
   py_ = make_axis_plan(ny);
   pz_ = make_axis_plan(nz);
+  big_x_ = uses_big_radix(px_);
+  big_y_ = uses_big_radix(py_);
+  big_z_ = uses_big_radix(pz_);
   const size_t cb = sizeof(cplx<T>);
   const size_t scale = sizeof(T) / sizeof(float);
   const size_t zbudget = kZBufBytes * scale, ybudget = kYBufBytes * scale,
@@ -634,7 +637,7 @@ void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const V
   const int P = pz_lines_;
   const int zblocks = zinv_blocks();
   const size_t zlds = (size_t(g_.nz) + 2 * size_t(zbuf_elems(P, g_.nz, g_.nzc))) * sizeof(cplx<T>);
-  auto kzf = k_z_forward<T, MODE, TIn>;
+  auto kzf = big_z_ ? k_z_forward<T, MODE, TIn, true> : k_z_forward<T, MODE, TIn, false>;
   allow_lds(kzf, zlds);
   profiler().launch(MODE == kModeMatvec ? kKZForward : residual_id(kKZForward), kzf, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, in, tangent, a, sgn, st, spec_);
   MTL_LAUNCH_CHECK();
@@ -642,7 +645,7 @@ void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const V
   const MinresState* stp = MODE == kModeMatvec ? st : nullptr;
   const int ytiles = ceil_div(g_.nzc, ty_);
   const size_t ylds = (size_t(g_.ny) + 2 * size_t(ty_) * g_.ny) * sizeof(cplx<T>);
-  auto ky = k_y_pass<T>;
+  auto ky = big_y_ ? k_y_pass<T, true> : k_y_pass<T, false>;
   allow_lds(ky, ylds);
   profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, false, stp, spec_);
   MTL_LAUNCH_CHECK();
@@ -652,12 +655,16 @@ void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const V
       x_resident_ ? (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<T>)
       : x_pairs_  ? (size_t(g_.nx) + 2 * size_t(tx_) * g_.nx) * sizeof(cplx<T>)
                   : x_lds_bytes(g_.nx, tx_, sizeof(cplx<T>));
-  auto kx = k_x_ghat_resident<T>;
+  auto kx = big_x_ ? k_x_ghat_resident<T, true> : k_x_ghat_resident<T, false>;
   if (!x_resident_) {
-    if constexpr (sizeof(T) == 4)
-      kx = x_pairs_ ? k_x_ghat_streamed<T, true> : k_x_ghat_streamed<T, false>;
-    else
-      kx = k_x_ghat_streamed<T, false>;
+    if constexpr (sizeof(T) == 4) {
+      if (x_pairs_)
+        kx = big_x_ ? k_x_ghat_streamed<T, true, true> : k_x_ghat_streamed<T, true, false>;
+      else
+        kx = big_x_ ? k_x_ghat_streamed<T, false, true> : k_x_ghat_streamed<T, false, false>;
+    } else {
+      kx = big_x_ ? k_x_ghat_streamed<T, false, true> : k_x_ghat_streamed<T, false, false>;
+    }
   }
   allow_lds(kx, xlds);
   profiler().launch(MODE == kModeMatvec ? kKXGhat : residual_id(kKXGhat), kx, dim3(g_.ny * xtiles), dim3(kXBT), xlds, s, g_, pxd_, twx_, tx_, xtiles, tanx_, tany_, tanz_, mask_bits(mask), stp, spec_);
@@ -666,7 +673,7 @@ void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const V
   profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, true, stp, spec_);
   MTL_LAUNCH_CHECK();
 
-  auto kzi = k_z_inverse<T, MODE, TOut>;
+  auto kzi = big_z_ ? k_z_inverse<T, MODE, TOut, true> : k_z_inverse<T, MODE, TOut, false>;
   allow_lds(kzi, zlds);
   profiler().launch(MODE == kModeMatvec ? kKZInverse : residual_id(kKZInverse), kzi, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, spec_, out, r1, reinterpret_cast<const T*>(in), st, partials);
   MTL_LAUNCH_CHECK();

@@ -53,6 +53,7 @@ class Spectral {
   int tx_;        // kz columns per workgroup in the x pass
   bool x_resident_;  // x pass keeps all six components of its tile in LDS
   bool x_pairs_ = false;  // streamed x pass moves pairs of complex values
+  bool big_x_ = false, big_y_ = false, big_z_ = false;  // axis plans use radix 11 or 13
   cplx<T>* spec_ = nullptr;
   cplx<T>* twx_ = nullptr;
   cplx<T>* twy_ = nullptr;

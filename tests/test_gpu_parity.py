@@ -30,7 +30,8 @@ def _need_gpu():
 # 250 x 3 x 4 and 256 x 3 x 32 (whole even-width tiles) move pairs of values.
 GRIDS = [(3, 4, 5), (4, 4, 4), (2, 3, 6), (1, 4, 3), (6, 5, 8), (8, 8, 8), (7, 9, 10),
          (16, 16, 16), (12, 10, 14), (32, 32, 32), (48, 48, 48),
-         (256, 4, 6), (256, 2, 40), (250, 3, 4), (512, 2, 6), (256, 3, 32)]
+         (256, 4, 6), (256, 2, 40), (250, 3, 4), (512, 2, 6), (256, 3, 32),
+         (11, 13, 4), (22, 26, 12), (13, 11, 26)]  # radix-11/13 passes on every axis
 
 
 @pytest.mark.parametrize("dims", GRIDS)
@@ -55,6 +56,15 @@ def test_projection_matches_oracle(dims, precision):
         scale = np.abs(ref).max()
         tol = 1e-13 if precision == "fp64" else 2e-6
         assert np.abs(got - ref).max() <= tol * scale, (dims, precision, np.abs(got - ref).max() / scale)
+
+
+def test_unsupported_prime_factor_is_an_error():
+    """Grid lengths with a prime factor above 13 are rejected with a message
+    (no silent fallback)."""
+    from materialite_amd._native import NativeSolver
+
+    with pytest.raises(Exception, match="prime factor above 13"):
+        NativeSolver((17, 4, 4), "fp64")
 
 
 @pytest.mark.parametrize("dims,precision", [((128, 128, 128), "fp64"), ((128, 128, 128), "mixed"),
