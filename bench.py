@@ -14,7 +14,13 @@ resident in HBM before the timed region.
 
 The roofline entry is the dominant kernel's algorithmic HBM bytes per launch
 divided by its mean launch time (HIP events recorded by the library on the
-launching stream during the timed region), against 8 TB/s.
+launching stream during the timed region), against 8 TB/s.  For scale the entry
+also carries the rate of a plain device-to-device copy of 512 MiB measured in
+the same process (torch's copy kernel, read + write bytes over its duration).
+
+fp64_form: the plain FP64 form on the same increments, the mixed form's stress
+error against it, and both forms' distance to an FP64 solution of the same
+increments converged to Newton 1e-6 / MINRES 1e-8 (converged_reference).
 
 cpu_baseline: the oracle (NumPy FP64 restatement of the reference path) timed
 on this host for a bounded number of MINRES iterations of the first Newton
@@ -57,6 +63,9 @@ def parse():
                     help="64^3 elastic, uniaxial x then xy shear: mixed vs FP64 stress error")
     ap.add_argument("--large", type=int, default=1,
                     help="also run the 256^3 two-phase EVP solve (mixed and FP64; about a minute)")
+    ap.add_argument("--large-converged", type=int, default=0,
+                    help="with --large: also solve the 256^3 case with FP64 at Newton 1e-6 / "
+                         "MINRES 1e-8 and report both forms' distance to it (a few minutes)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB copied by a plain device copy before the workload (the reference "
                          "dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
@@ -145,6 +154,9 @@ def _timed_run(material, schedule, model, precision, end_time, dt, **run_kw):
         else:
             rec["krylov"] += info["krylov_iterations"]
         rec["t1"] = time.perf_counter()
+        if info["step"] % 5 == 0:
+            print(f"[bench]   {precision} increment {info['step']}, {rec['krylov']} Krylov",
+                  file=sys.stderr, flush=True)
         return False
 
     solver = SmallStrainFFT(schedule, end_time=end_time, initial_time_increment=dt,
@@ -187,13 +199,14 @@ def precision_study(n=64, grains=200, steps=10):
             "max_rel_stress_error": float(np.abs(sm - sd).max() / np.abs(sd).max()),
             "krylov_mixed": int(km), "krylov_fp64": int(kd),
             "vi_per_s_mixed": vim, "vi_per_s_fp64": vid,
+            "effective_vi_per_s_mixed": n ** 3 * kd / tm,
             "gb_per_s_mixed": vim * bytes_per_vi(n, "mixed") / 1e9,
             "gb_per_s_fp64": vid * bytes_per_vi(n, "fp64") / 1e9,
         }
     return out
 
 
-def large_two_phase(n=256, grains=1000, steps=30, dt=0.2):
+def large_two_phase(n=256, grains=1000, steps=30, dt=0.2, converged=False):
     """256^3 two-phase EVP: 1000-grain FCC matrix (seed 20260005) with 10 %
     volume fraction of periodic non-overlapping spheres of radius 8, elastic
     isotropic with K, mu = 10 x the matrix Voigt averages; uniaxial z 1e-3/s,
@@ -228,14 +241,26 @@ def large_two_phase(n=256, grains=1000, steps=30, dt=0.2):
         res[prec] = _timed_run(m, ls, None, prec, steps * dt, dt, phase_label="phase")
     (sm, km, tm), (sd, kd, td) = res["mixed"], res["fp64"]
     vim, vid = n ** 3 * km / tm, n ** 3 * kd / td
+    conv = None
+    if converged:
+        print("[bench] 256^3 two-phase: fp64, Newton 1e-6 / MINRES 1e-8", file=sys.stderr,
+              flush=True)
+        tight_kw = {"global_tolerance": 1e-6, "linear_solver_tolerance": 1e-8}
+        st, kt, _ = _timed_run(m, ls, None, "fp64", steps * dt, dt, phase_label="phase", **tight_kw)
+        scale = np.abs(st).max()
+        conv = dict(tight_kw, krylov_iterations=int(kt),
+                    max_rel_stress_error_of_fp64=float(np.abs(sd - st).max() / scale),
+                    max_rel_stress_error_of_mixed=float(np.abs(sm - st).max() / scale))
     return {
         "workload": f"{n}^3 two-phase EVP, {grains} grains, {len(centres)} spheres r 8 "
                     f"({inclusion.mean():.3f} vf), uniaxial z, {steps} x {dt} s",
         "max_rel_stress_error": float(np.abs(sm - sd).max() / np.abs(sd).max()),
         "krylov_mixed": int(km), "krylov_fp64": int(kd),
         "vi_per_s_mixed": vim, "vi_per_s_fp64": vid,
+        "effective_vi_per_s_mixed": n ** 3 * kd / tm,
         "ms_per_step_mixed": 1e3 * tm / (steps - 1), "ms_per_step_fp64": 1e3 * td / (steps - 1),
         "roofline_frac_mixed": vim * bytes_per_vi(n, "mixed") / (HBM_PEAK_GBS * 1e9),
+        "converged_reference": conv,
     }
 
 
@@ -272,7 +297,7 @@ def evp_model():
                                                      "theta_1": 0.0})
 
 
-def time_increments(material, precision, warmup, steps, profile):
+def time_increments(material, precision, warmup, steps, profile, **run_kw):
     from materialite_amd import _native
     from materialite_amd.models.small_strain_fft import LoadSchedule, SmallStrainFFT
 
@@ -303,7 +328,7 @@ def time_increments(material, precision, warmup, steps, profile):
 
     if warmup == 0:
         raise SystemExit("--warmup must be >= 1 (the first increment builds the solver state)")
-    out = model.run(material, increment_callback=cb)
+    out = model.run(material, increment_callback=cb, **run_kw)
     rec["stress"] = out.extract("stress").components
     if profile:
         rec["profile"] = _native.profile_read()
@@ -561,6 +586,25 @@ def cpu_baseline(n, grains, iters):
     }
 
 
+def copy_ceiling(mib=512, reps=10):
+    """Read+write GB/s of torch's device-to-device copy of `mib` MiB (FP32)."""
+    import torch
+
+    src = torch.ones(mib * 2 ** 18, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(3):
+        dst.copy_(src)
+    t0, t1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    t0.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    t1.record()
+    t1.synchronize()
+    sec = t0.elapsed_time(t1) / 1e3 / reps
+    del src, dst
+    return 2 * mib * 2 ** 20 / sec / 1e9
+
+
 def main():
     args = parse()
     if args.cpu_baseline_child:
@@ -608,6 +652,7 @@ def main():
     working = durs[durs > 0.2 * durs.max()]
     mean_s = float(working.mean()) / 1e3
     achieved = kb[best] / mean_s / 1e9
+    copy_gbs = copy_ceiling()
     fp64 = None
     if args.compare_fp64 and args.precision != "fp64":
         rec = time_increments(material, "fp64", args.warmup, args.steps, profile=False)
@@ -615,14 +660,30 @@ def main():
             "max_rel_stress_error_of_mixed": float(
                 abs(main_rec["stress"] - rec["stress"]).max() / abs(rec["stress"]).max()),
             "value": npts * rec["krylov"] / rec["seconds"],
+            # the mixed form's rate counted with this form's iteration count
+            "effective_value_mixed": npts * rec["krylov"] / main_rec["seconds"],
             "krylov_iterations": rec["krylov"],
             "seconds": rec["seconds"],
             "ms_per_step": 1e3 * rec["seconds"] / max(rec["steps"], 1),
         }
+        # both forms against a tightly converged FP64 solution of the same
+        # increments: at the default tolerances (Newton 1e-3, MINRES 1e-5) each
+        # form stops at its own iterate, and this shows how far each is from
+        # the converged state
+        tight_kw = {"global_tolerance": 1e-6, "linear_solver_tolerance": 1e-8}
+        tight = time_increments(material, "fp64", args.warmup, args.steps, profile=False,
+                                **tight_kw)
+        scale = abs(tight["stress"]).max()
+        fp64["converged_reference"] = dict(
+            tight_kw,
+            krylov_iterations=tight["krylov"],
+            max_rel_stress_error_of_fp64=float(abs(rec["stress"] - tight["stress"]).max() / scale),
+            max_rel_stress_error_of_mixed=float(
+                abs(main_rec["stress"] - tight["stress"]).max() / scale))
     stream = (serving_stream(args.stream_requests, args.stream_rate, args.stream_slots)
               if args.stream_requests > 0 else None)
     study = precision_study() if args.precision_study else None
-    large = large_two_phase() if args.large else None
+    large = large_two_phase(converged=bool(args.large_converged)) if args.large else None
     cpu = cpu_baseline(n, args.grains, args.cpu_iterations) if args.cpu_baseline else None
     c1 = config1() if args.config1 else None
     line = {
@@ -656,6 +717,8 @@ def main():
             "traffic": measured_traffic(best) if args.precision == "mixed" and n == 128 else None,
             "mean_launch_us": mean_s * 1e6,
             "timed_launches": int(working.size),
+            "copy_gbs": copy_gbs,
+            "frac_of_copy": achieved / copy_gbs,
         },
         "kernels_ms": {k: round(v[0], 3) for k, v in prof.items()},
         "kernels_launches": {k: int(v[1]) for k, v in prof.items()},

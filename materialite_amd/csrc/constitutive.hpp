@@ -54,6 +54,16 @@ template <typename K>
 std::unique_ptr<Model<K>> make_model(const ModelDesc& d, const std::vector<int>& points,
                                      long long npts);
 
+// slip-system counts the elasto-viscoplastic kernels are compiled for
+inline bool evp_slip_count_built(int ns) {
+  switch (ns) {
+    case 1: case 2: case 3: case 4: case 6: case 9: case 12: case 15: case 18: case 24:
+      return true;
+    default:
+      return false;
+  }
+}
+
 enum FieldId {
   kFieldStress = 0,
   kFieldStrain = 1,

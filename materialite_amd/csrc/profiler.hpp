@@ -11,6 +11,8 @@
 
 #include <hip/hip_ext.h>
 
+#include <atomic>
+#include <mutex>
 #include <vector>
 
 #include "common.hpp"
@@ -43,20 +45,19 @@ enum KernelId {
 // matvec pass id -> the residual-projection id of the same pass
 constexpr int residual_id(int matvec_id) { return matvec_id + kKResidualZForward; }
 
+// Handles used from several threads at once (the serving layer) share the
+// profiler: with timing off a launch only reads an atomic flag; with timing on the
+// records are kept under a mutex.
 class Profiler {
  public:
-  bool enabled = false;
-  double total_ms[kNumKernelIds] = {};
-  long long launches[kNumKernelIds] = {};
-  std::vector<float> durations[kNumKernelIds];  // ms of each timed launch
-
   template <typename F, typename... Args>
   void launch(int id, F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s, Args... args) {
-    if (!enabled) {
+    if (!enabled_.load(std::memory_order_relaxed)) {
       hipExtLaunchKernelGGL(kernel, grid, block, std::uint32_t(lds), s, nullptr, nullptr, 0u,
                             args...);
       return;
     }
+    std::lock_guard<std::mutex> lock(mu_);
     if (wake_next_) {
       wake_next_ = false;
       hipLaunchKernelGGL(profiler_wake_kernel, dim3(1), dim3(64), 0, s);
@@ -64,33 +65,56 @@ class Profiler {
     hipEvent_t a = take(), b = take();
     hipExtLaunchKernelGGL(kernel, grid, block, std::uint32_t(lds), s, a, b, 0u, args...);
     pending_.push_back({id, a, b});
-    if (pending_.size() > 4096) flush();
+    if (pending_.size() > 4096) flush_locked();
   }
 
   // the next launch follows a host synchronisation
-  void after_sync() { wake_next_ = enabled; }
+  void after_sync() {
+    if (!enabled_.load(std::memory_order_relaxed)) return;
+    std::lock_guard<std::mutex> lock(mu_);
+    wake_next_ = true;
+  }
+
+  void set_enabled(bool on) {
+    std::lock_guard<std::mutex> lock(mu_);
+    flush_locked();
+    enabled_.store(on, std::memory_order_relaxed);
+    wake_next_ = false;
+  }
 
   void flush() {
-    for (auto& r : pending_) {
-      MTL_HIP(hipEventSynchronize(r.b));
-      float ms = 0.f;
-      MTL_HIP(hipEventElapsedTime(&ms, r.a, r.b));
-      total_ms[r.id] += ms;
-      launches[r.id] += 1;
-      durations[r.id].push_back(ms);
-      pool_.push_back(r.a);
-      pool_.push_back(r.b);
-    }
-    pending_.clear();
+    std::lock_guard<std::mutex> lock(mu_);
+    flush_locked();
   }
+
   void reset() {
-    flush();
+    std::lock_guard<std::mutex> lock(mu_);
+    flush_locked();
     for (int i = 0; i < kNumKernelIds; ++i) {
-      total_ms[i] = 0.0;
-      launches[i] = 0;
-      durations[i].clear();
+      total_ms_[i] = 0.0;
+      launches_[i] = 0;
+      durations_[i].clear();
     }
     wake_next_ = false;
+  }
+
+  // totals of kernel id `id` (after waiting for its pending records)
+  void read(int id, double* total_ms, long long* launches) {
+    std::lock_guard<std::mutex> lock(mu_);
+    flush_locked();
+    *total_ms = total_ms_[id];
+    *launches = launches_[id];
+  }
+
+  // durations (ms) of the timed launches of `id`: copies up to `capacity`,
+  // returns how many there are
+  long long read_durations(int id, float* out, long long capacity) {
+    std::lock_guard<std::mutex> lock(mu_);
+    flush_locked();
+    const auto& d = durations_[id];
+    const long long m = capacity < (long long)d.size() ? capacity : (long long)d.size();
+    for (long long i = 0; i < m; ++i) out[i] = d[size_t(i)];
+    return (long long)d.size();
   }
 
  private:
@@ -98,6 +122,19 @@ class Profiler {
     int id;
     hipEvent_t a, b;
   };
+  void flush_locked() {
+    for (auto& r : pending_) {
+      MTL_HIP(hipEventSynchronize(r.b));
+      float ms = 0.f;
+      MTL_HIP(hipEventElapsedTime(&ms, r.a, r.b));
+      total_ms_[r.id] += ms;
+      launches_[r.id] += 1;
+      durations_[r.id].push_back(ms);
+      pool_.push_back(r.a);
+      pool_.push_back(r.b);
+    }
+    pending_.clear();
+  }
   hipEvent_t take() {
     if (pool_.empty()) {
       hipEvent_t e;
@@ -108,7 +145,12 @@ class Profiler {
     pool_.pop_back();
     return e;
   }
+  std::atomic<bool> enabled_{false};
+  std::mutex mu_;
   bool wake_next_ = false;
+  double total_ms_[kNumKernelIds] = {};
+  long long launches_[kNumKernelIds] = {};
+  std::vector<float> durations_[kNumKernelIds];  // ms of each timed launch
   std::vector<Rec> pending_;
   std::vector<hipEvent_t> pool_;
 };

@@ -260,6 +260,9 @@ class Solver final : public SolverBase {
   int add_model(const ModelDesc& d) override {
     if (d.kind == 1 && (d.nslip < 1 || int(d.schmid.size()) != 6 * d.nslip))
       throw Error("bad slip-system table");
+    if (d.kind == 1 && !evp_slip_count_built(d.nslip))
+      throw Error("elasto-viscoplastic model with " + std::to_string(d.nslip) +
+                  " slip systems is not built (supported: 1, 2, 3, 4, 6, 9, 12, 15, 18, 24)");
     descs_.push_back(d);
     return int(descs_.size()) - 1;
   }
@@ -699,6 +702,7 @@ int mtl_solver_add_evp(mtl_solver* h, const double stiffness[36], int num_slip,
                        int* model_id) {
   if (!h || !stiffness || !schmid || !model_id) return bad("null argument");
   if (hardening < 0 || hardening > 3) return bad("unknown hardening law");
+  if (num_slip < 1) return bad("EVP model needs at least one slip system");
   return guarded([&] {
     mtl::ModelDesc d;
     d.kind = 1;
@@ -777,10 +781,7 @@ int mtl_solver_minres(mtl_solver* h, const double* b_dev, double* x_dev,
 }
 
 int mtl_profile_enable(int enable) {
-  return guarded([&] {
-    mtl::profiler().flush();
-    mtl::profiler().enabled = enable != 0;
-  });
+  return guarded([&] { mtl::profiler().set_enabled(enable != 0); });
 }
 
 int mtl_profile_reset(void) {
@@ -790,23 +791,13 @@ int mtl_profile_reset(void) {
 int mtl_profile_read(int kernel, double* total_ms, long long* launches) {
   if (!total_ms || !launches) return bad("null argument");
   if (kernel < 0 || kernel >= mtl::kNumKernelIds) return bad("unknown kernel id");
-  return guarded([&] {
-    mtl::profiler().flush();
-    *total_ms = mtl::profiler().total_ms[kernel];
-    *launches = mtl::profiler().launches[kernel];
-  });
+  return guarded([&] { mtl::profiler().read(kernel, total_ms, launches); });
 }
 
 int mtl_profile_durations(int kernel, float* ms_out, long long capacity, long long* count) {
   if (!count || (capacity > 0 && !ms_out)) return bad("null argument");
   if (kernel < 0 || kernel >= mtl::kNumKernelIds) return bad("unknown kernel id");
-  return guarded([&] {
-    mtl::profiler().flush();
-    const auto& d = mtl::profiler().durations[kernel];
-    *count = (long long)d.size();
-    const long long m = std::min<long long>(capacity, (long long)d.size());
-    for (long long i = 0; i < m; ++i) ms_out[i] = d[size_t(i)];
-  });
+  return guarded([&] { *count = mtl::profiler().read_durations(kernel, ms_out, capacity); });
 }
 
 int mtl_solver_constitutive(mtl_solver* h, const double* strain_dev, const double* guess_dev,

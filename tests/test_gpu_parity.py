@@ -67,6 +67,26 @@ def test_unsupported_prime_factor_is_an_error():
         NativeSolver((17, 4, 4), "fp64")
 
 
+def test_invalid_model_arguments_are_errors():
+    """Model registration rejects what the kernels cannot run, at the call
+    that passes it: no slip systems, a slip-system count the EVP kernels are
+    not compiled for, an unknown hardening law."""
+    from materialite_amd._native import NativeSolver
+
+    sol = NativeSolver((4, 4, 4), "fp64")
+    try:
+        c = np.eye(6) * 100.0
+        with pytest.raises(Exception, match="at least one slip system"):
+            sol.add_evp(c, np.zeros((0, 6)), 1e-3, 20.0, 50.0, 0, [])
+        with pytest.raises(Exception, match="5 slip systems is not built"):
+            sol.add_evp(c, np.ones((5, 6)), 1e-3, 20.0, 50.0, 0, [])
+        with pytest.raises(Exception, match="unknown hardening law"):
+            sol.add_evp(c, np.ones((12, 6)), 1e-3, 20.0, 50.0, 7, [])
+        assert sol.add_evp(c, np.ones((12, 6)), 1e-3, 20.0, 50.0, 0, []) == 0
+    finally:
+        sol.close()
+
+
 @pytest.mark.parametrize("dims,precision", [((128, 128, 128), "fp64"), ((128, 128, 128), "mixed"),
                                              ((256, 256, 256), "mixed")])
 def test_projection_properties_full_size(dims, precision):
