@@ -253,9 +253,14 @@ class NativeSolver:
         check(self._lib.mtl_solver_rollback(self._h))
 
     def get_field(self, name, ncomp_hint=36):
-        out = np.empty((self.num_points, max(ncomp_hint, 36)), dtype=np.float64)
+        # stress, strain and plastic strain always have 6 columns; other fields
+        # get room for the widest one (36, the tangent) and are cut to size
+        width = 6 if name in ("stress", "strain", "plastic_strains") else max(ncomp_hint, 36)
+        out = np.empty((self.num_points, width), dtype=np.float64)
         nc = ctypes.c_int()
         check(self._lib.mtl_solver_get_field(self._h, FIELD[name], _ptr(out), ctypes.byref(nc)))
+        if nc.value == width:
+            return out
         flat = out.reshape(-1)[: self.num_points * nc.value]
         return flat.reshape(self.num_points, nc.value).copy()
 

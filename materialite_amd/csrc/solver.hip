@@ -193,6 +193,28 @@ __global__ void k_convert(long long count, const From* a, To* b) {
     b[i] = To(a[i]);
 }
 
+// point-major copy of an SoA field: aos[p * nc + c] = soa[c * n + p]
+__global__ void k_soa_to_aos(long long n, int nc, const double* soa, double* aos) {
+  const long long count = n * nc;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long p = i / nc;
+    const int c = int(i - p * nc);
+    aos[i] = soa[c * n + p];
+  }
+}
+
+// SoA copy of a point-major field: soa[c * n + p] = aos[p * nc + c]
+__global__ void k_aos_to_soa(long long n, int nc, const double* aos, double* soa) {
+  const long long count = n * nc;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long p = i / nc;
+    const int c = int(i - p * nc);
+    soa[c * n + p] = aos[i];
+  }
+}
+
 constexpr double kMixedKrylovFloor = 1.0e-5;
 
 bool debug_enabled() {
@@ -268,10 +290,17 @@ class Solver final : public SolverBase {
   }
 
   void set_microstructure(const double* rot, const int32_t* mop) override {
-    std::vector<double> soa(9 * size_t(n_));
-    for (long long p = 0; p < n_; ++p)
-      for (int k = 0; k < 9; ++k) soa[size_t(k) * n_ + p] = rot[p * 9 + k];
-    MTL_HIP(hipMemcpy(rot_, soa.data(), soa.size() * sizeof(double), hipMemcpyHostToDevice));
+    // point-major rotations are uploaded as they are and reordered to SoA on
+    // the device (a stream-ordered temporary: no device-wide synchronisation)
+    double* aos = nullptr;
+    const size_t bytes = 9 * size_t(n_) * sizeof(double);
+    MTL_HIP(hipMallocAsync(reinterpret_cast<void**>(&aos), bytes, s_));
+    MTL_HIP(hipMemcpyAsync(aos, rot, bytes, hipMemcpyHostToDevice, s_));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(9 * n_)), dim3(kNBT), 0, s_, n_, 9,
+                       static_cast<const double*>(aos), rot_);
+    MTL_LAUNCH_CHECK();
+    MTL_HIP(hipFreeAsync(aos, s_));
+    stream_sync(s_);
     std::vector<std::vector<int>> pts(descs_.size());
     for (long long p = 0; p < n_; ++p) {
       const int m = mop ? mop[p] : 0;
@@ -575,12 +604,14 @@ class Solver final : public SolverBase {
   void restore_tangent() {
     MTL_HIP(hipMemcpyAsync(D_, D_old_, 21 * n_ * sizeof(K), hipMemcpyDeviceToDevice, s_));
   }
+  // Reorders on the device into b_ (the Newton residual, rewritten before it
+  // is read in every step) and copies the point-major result to `out`.
   void soa_to_host(const double* d, int nc, double* out) {
-    std::vector<double> soa(size_t(nc) * n_);
-    MTL_HIP(hipMemcpyAsync(soa.data(), d, soa.size() * sizeof(double), hipMemcpyDeviceToHost, s_));
+    if (nc > 6 || d == b_) throw Error("soa_to_host: bad field");
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(nc * n_)), dim3(kNBT), 0, s_, n_, nc, d, b_);
+    MTL_LAUNCH_CHECK();
+    MTL_HIP(hipMemcpyAsync(out, b_, size_t(nc) * n_ * sizeof(double), hipMemcpyDeviceToHost, s_));
     stream_sync(s_);
-    for (long long p = 0; p < n_; ++p)
-      for (int c = 0; c < nc; ++c) out[p * nc + c] = soa[size_t(c) * n_ + p];
   }
 
   Spectral<double> sp64_;
