@@ -847,16 +847,17 @@ class EvpModel final : public Model<K> {
     switch (which) {
       case kFieldPlasticStrain: {
         // committed crystal plastic strain, rotated to the specimen frame
+        // stream-ordered temporary: no device-wide synchronisation
         double* tmp = nullptr;
-        MTL_HIP(hipMalloc(&tmp, std::max(6 * n, 1) * sizeof(double)));
+        MTL_HIP(hipMallocAsync(reinterpret_cast<void**>(&tmp), std::max(6 * n, 1) * sizeof(double), s));
         if (n)
           hipLaunchKernelGGL(k_ep_specimen, dim3(ceil_div(n, kCBT)), dim3(kCBT), 0, s, n,
                              this->idx_, this->npts_, rot9, slot_[ring_.old].ep, tmp);
         out.resize(size_t(6) * n);
         if (n) MTL_HIP(hipMemcpyAsync(out.data(), tmp, out.size() * sizeof(double),
                                       hipMemcpyDeviceToHost, s));
+        MTL_HIP(hipFreeAsync(tmp, s));
         stream_sync(s);
-        (void)hipFree(tmp);
         ncomp = 6;
         return true;
       }
