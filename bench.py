@@ -20,7 +20,9 @@ the same process (torch's copy kernel, read + write bytes over its duration).
 
 fp64_form: the plain FP64 form on the same increments, the mixed form's stress
 error against it, and both forms' distance to an FP64 solution of the same
-increments converged to Newton 1e-6 / MINRES 1e-8 (converged_reference).
+increments converged to Newton 1e-6 / MINRES 1e-8 (converged_reference); also
+the "mixed-fft64" form (the mixed form's FP32 data with FP64 arithmetic inside
+the FFT passes) on the same increments (mixed_fft64_form).
 
 cpu_baseline: the oracle (NumPy FP64 restatement of the reference path) timed
 on this host for a bounded number of MINRES iterations of the first Newton
@@ -49,7 +51,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--grains", type=int, default=1000)
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "fp64"])
+    ap.add_argument("--precision", default="mixed", choices=["mixed", "mixed-fft64", "fp64"])
     ap.add_argument("--compare-fp64", type=int, default=1,
                     help="also time the plain FP64 form on the same increments")
     ap.add_argument("--cpu-baseline", type=int, default=1)
@@ -136,7 +138,7 @@ def bytes_per_vi(nz, precision):
     """Algorithmic HBM bytes per voxel-iteration of the minimal fused pipeline:
     tangent (21), Krylov input and output (6 + 6), four passes over the
     6-component half spectrum, and eight Mandel vectors of MINRES recurrences."""
-    k = 4 if precision == "mixed" else 8
+    k = 8 if precision == "fp64" else 4
     h = (nz // 2 + 1) / nz
     return 21 * k + 12 * k + 4 * 6 * 2 * k * h + 8 * 6 * k
 
@@ -339,8 +341,8 @@ def time_increments(material, precision, warmup, steps, profile, **run_kw):
 
 
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_evp128_mixed.txt")
-KERNEL_SYMBOLS = {"z_forward": "k_z_forward<float, 1, float", "y_pass": "k_y_pass<float",
-                  "x_ghat": r"k_x_ghat_\w+<float", "z_inverse": "k_z_inverse<float, 1, float",
+KERNEL_SYMBOLS = {"z_forward": "k_z_forward<float, float, 1, float", "y_pass": "k_y_pass<float, float",
+                  "x_ghat": r"k_x_ghat_\w+<float, float", "z_inverse": "k_z_inverse<float, float, 1, float",
                   "evp_iteration": "k_evp_iter<12, true"}
 
 
@@ -426,7 +428,7 @@ def measured_traffic(kernel):
 
 def kernel_bytes(n, nzc_frac, precision):
     """Algorithmic HBM bytes per launch of each pass kernel (FFT/Krylov type)."""
-    esz = 4 if precision == "mixed" else 8
+    esz = 8 if precision == "fp64" else 4
     spec = 6 * n * nzc_frac * 2 * esz  # one read or write of the 6-component half spectrum
     return {
         "z_forward": n * (6 + 21) * esz + spec,  # r2 (6) and tangent (21) read, spectrum written
@@ -680,6 +682,18 @@ def main():
             max_rel_stress_error_of_fp64=float(abs(rec["stress"] - tight["stress"]).max() / scale),
             max_rel_stress_error_of_mixed=float(
                 abs(main_rec["stress"] - tight["stress"]).max() / scale))
+        if args.precision == "mixed":
+            # the same FP32 data flow with FP64 arithmetic inside the FFT passes
+            f64a = time_increments(material, "mixed-fft64", args.warmup, args.steps, profile=False)
+            fp64["mixed_fft64_form"] = {
+                "value": npts * f64a["krylov"] / f64a["seconds"],
+                "ms_per_step": 1e3 * f64a["seconds"] / max(f64a["steps"], 1),
+                "krylov_iterations": f64a["krylov"],
+                "max_rel_stress_error": float(
+                    abs(f64a["stress"] - rec["stress"]).max() / abs(rec["stress"]).max()),
+                "max_rel_stress_error_vs_converged": float(
+                    abs(f64a["stress"] - tight["stress"]).max() / scale),
+            }
     stream = (serving_stream(args.stream_requests, args.stream_rate, args.stream_slots)
               if args.stream_requests > 0 else None)
     study = precision_study() if args.precision_study else None
@@ -697,7 +711,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f32" if args.precision == "mixed" else "f64",
+        "dtype": "f64" if args.precision == "fp64" else "f32",
         "data": "synthetic",
         "config": {
             "workload": f"{n}^3 EVP FCC, {args.grains} periodic Voronoi grains, Voce, uniaxial x "

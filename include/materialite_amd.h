@@ -29,8 +29,10 @@ typedef struct mtl_solver mtl_solver;
 
 /* precision of a solver handle */
 enum { MTL_FP64 = 0, /* every field, FFT and Krylov vector in FP64 */
-       MTL_MIXED = 1 /* Krylov vectors, tangents and matvec FFTs in FP32;
-                        strain/stress, residual FFT and all norms in FP64 */ };
+       MTL_MIXED = 1, /* Krylov vectors, tangents and matvec FFTs in FP32;
+                         strain/stress, residual FFT and all norms in FP64 */
+       MTL_MIXED_FFT64 = 2 /* as MTL_MIXED (FP32 data in device memory), with the
+                              matvec FFT butterflies, twiddles and Ghat in FP64 */ };
 
 /* hardening laws (small_strain_fft/hardening_laws.py) */
 enum { MTL_HARDENING_PERFECT = 0, /* :18-25 */

@@ -16,6 +16,8 @@ LIB_PATH = os.environ.get("MATERIALITE_AMD_LIB", os.path.join(HERE, "libmaterial
 
 MTL_FP64 = 0
 MTL_MIXED = 1
+MTL_MIXED_FFT64 = 2
+PRECISIONS = {"fp64": MTL_FP64, "mixed": MTL_MIXED, "mixed-fft64": MTL_MIXED_FFT64}
 
 HARDENING = {
     "perfect_plasticity": 0,
@@ -184,7 +186,9 @@ class NativeSolver:
 
     def __init__(self, dims, precision="fp64"):
         self._lib = load_library()
-        prec = {"fp64": MTL_FP64, "mixed": MTL_MIXED}[precision]
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, not {precision!r}")
+        prec = PRECISIONS[precision]
         h = ctypes.c_void_p()
         nx, ny, nz = (int(d) for d in dims)
         check(self._lib.mtl_solver_create(nx, ny, nz, prec, ctypes.byref(h)))

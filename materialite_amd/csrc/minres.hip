@@ -197,8 +197,8 @@ int vec_blocks(long long n6) {
 
 }  // namespace
 
-template <typename T>
-Minres<T>::Minres(Spectral<T>& sp) : sp_(sp) {
+template <typename T, typename A>
+Minres<T, A>::Minres(Spectral<T, A>& sp) : sp_(sp) {
   n6_ = 6 * sp.grid().n;
   for (auto& v : vec_) MTL_HIP(hipMalloc(&v, n6_ * sizeof(T)));
   MTL_HIP(hipMalloc(&x_, n6_ * sizeof(T)));
@@ -208,8 +208,8 @@ Minres<T>::Minres(Spectral<T>& sp) : sp_(sp) {
   MTL_HIP(hipHostMalloc(&host_state_, sizeof(MinresState), hipHostMallocDefault));
 }
 
-template <typename T>
-Minres<T>::~Minres() {
+template <typename T, typename A>
+Minres<T, A>::~Minres() {
   for (auto v : vec_)
     if (v) (void)hipFree(v);
   if (x_) (void)hipFree(x_);
@@ -218,8 +218,8 @@ Minres<T>::~Minres() {
   if (host_state_) (void)hipHostFree(host_state_);
 }
 
-template <typename T>
-int Minres<T>::solve(const double* b, const T* tangent, const Vec6& mask, double rtol,
+template <typename T, typename A>
+int Minres<T, A>::solve(const double* b, const T* tangent, const Vec6& mask, double rtol,
                      long long maxiter, hipStream_t s, int* istop_out) {
   // roles: r1, r2 (Lanczos), w1, w2 (previous w), wf (free slot for the new w)
   T* r1 = vec_[0];
@@ -288,5 +288,6 @@ int Minres<T>::solve(const double* b, const T* tangent, const Vec6& mask, double
 
 template class Minres<float>;
 template class Minres<double>;
+template class Minres<float, double>;
 
 }  // namespace mtl

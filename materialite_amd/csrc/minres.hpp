@@ -5,10 +5,11 @@
 
 namespace mtl {
 
-template <typename T>
+// T: Krylov vectors and tangent; A: arithmetic type of the matvec FFT passes.
+template <typename T, typename A = T>
 class Minres {
  public:
-  explicit Minres(Spectral<T>& sp);
+  explicit Minres(Spectral<T, A>& sp);
   ~Minres();
   Minres(const Minres&) = delete;
   Minres& operator=(const Minres&) = delete;
@@ -23,7 +24,7 @@ class Minres {
   const MinresState& state() const { return *host_state_; }
 
  private:
-  Spectral<T>& sp_;
+  Spectral<T, A>& sp_;
   long long n6_;
   T* vec_[5] = {};  // two Lanczos vectors and three w vectors (roles rotate)
   T* x_ = nullptr;

@@ -215,7 +215,10 @@ __global__ void k_aos_to_soa(long long n, int nc, const double* aos, double* soa
   }
 }
 
+// lowest relative MINRES tolerance of the mixed forms: FP32 vectors with FP32
+// FFT arithmetic, and with FP64 FFT arithmetic
 constexpr double kMixedKrylovFloor = 1.0e-5;
+constexpr double kMixedFft64KrylovFloor = 1.0e-6;
 
 bool debug_enabled() {
   static const bool on = [] {
@@ -247,17 +250,18 @@ struct SolverBase {
   virtual int constitutive(const double* eps, const double* guess, double dt, double* sig) = 0;
 };
 
-template <typename K>
+// K: tangent and Krylov vectors; A: arithmetic of the matvec FFT passes.
+template <typename K, typename A = K>
 class Solver final : public SolverBase {
  public:
   Solver(int nx, int ny, int nz) : sp64_(nx, ny, nz) {
-    if constexpr (std::is_same<K, double>::value) {
-      spk_ = &sp64_;
+    if constexpr (std::is_same<Spectral<K, A>, Spectral<double>>::value) {
+      spk_ = &sp64_;  // one spectrum buffer for the residual and the matvec
     } else {
-      own_spk_ = std::make_unique<Spectral<K>>(nx, ny, nz);
+      own_spk_ = std::make_unique<Spectral<K, A>>(nx, ny, nz);
       spk_ = own_spk_.get();
     }
-    minres_ = std::make_unique<Minres<K>>(*spk_);
+    minres_ = std::make_unique<Minres<K, A>>(*spk_);
     n_ = sp64_.grid().n;
     MTL_HIP(hipStreamCreate(&s_));
     auto dalloc = [&](double*& p, long long c) { MTL_HIP(hipMalloc(&p, c * sizeof(double))); };
@@ -420,14 +424,17 @@ class Solver final : public SolverBase {
 
     const int kMaxNewton = 100;  // small_strain_fft.py:125
     const long long maxiter = 5LL * 6LL * n_;
-    // Mixed form: FP32 Krylov solves cannot honour relative tolerances much
-    // below 1e-5, so the inner tolerance is floored there.  The Newton
-    // residual, the equilibrium test and the strain/stress fields stay FP64,
-    // so each Newton iteration is an FP64-residual refinement step and the
-    // converged state is the FP64 one up to the requested global tolerance.
+    // Mixed forms: FP32 Krylov solves cannot honour relative tolerances much
+    // below 1e-5 (1e-6 with FP64 FFT arithmetic) on the singular pore
+    // systems, so the inner tolerance is floored there.  The Newton residual,
+    // the equilibrium test and the strain/stress fields stay FP64, so each
+    // Newton iteration is an FP64-residual refinement step and the converged
+    // state is the FP64 one up to the requested global tolerance.
     constexpr bool kMixed = !std::is_same<K, double>::value;
+    constexpr double kFloor =
+        std::is_same<A, double>::value ? kMixedFft64KrylovFloor : kMixedKrylovFloor;
     const double lin_tol =
-        kMixed ? std::max(P.linear_solver_tolerance, kMixedKrylovFloor) : P.linear_solver_tolerance;
+        kMixed ? std::max(P.linear_solver_tolerance, kFloor) : P.linear_solver_tolerance;
     bool converged = false;
     int iteration = 0;
     int max_constit = 0;
@@ -615,9 +622,9 @@ class Solver final : public SolverBase {
   }
 
   Spectral<double> sp64_;
-  std::unique_ptr<Spectral<K>> own_spk_;
-  Spectral<K>* spk_ = nullptr;
-  std::unique_ptr<Minres<K>> minres_;
+  std::unique_ptr<Spectral<K, A>> own_spk_;
+  Spectral<K, A>* spk_ = nullptr;
+  std::unique_ptr<Minres<K, A>> minres_;
   long long n_ = 0;
   hipStream_t s_ = nullptr;
   double *eps_ = nullptr, *sig_ = nullptr, *sig_old_ = nullptr, *eps_old_ = nullptr,
@@ -702,13 +709,16 @@ const char* mtl_last_error(void) { return mtl::g_last_error.c_str(); }
 
 int mtl_solver_create(int nx, int ny, int nz, int precision, mtl_solver** out) {
   if (!out) return bad("null output handle");
-  if (precision != MTL_FP64 && precision != MTL_MIXED) return bad("unknown precision");
+  if (precision != MTL_FP64 && precision != MTL_MIXED && precision != MTL_MIXED_FFT64)
+    return bad("unknown precision");
   return guarded([&] {
     auto h = std::make_unique<mtl_solver>();
     if (precision == MTL_FP64)
       h->impl = std::make_unique<mtl::Solver<double>>(nx, ny, nz);
-    else
+    else if (precision == MTL_MIXED)
       h->impl = std::make_unique<mtl::Solver<float>>(nx, ny, nz);
+    else
+      h->impl = std::make_unique<mtl::Solver<float, double>>(nx, ny, nz);
     *out = h.release();
   });
 }

@@ -63,12 +63,39 @@ __device__ __forceinline__ void load_twiddles(cplx<T>* dst, const cplx<T>* src, 
   for (int i = threadIdx.x; i < L; i += blockDim.x) dst[i] = src[i];
 }
 
+// Storage type TS (the spectrum and Krylov vectors in HBM) and arithmetic type
+// T (LDS, twiddles, butterflies) may differ: FP32 data transformed in FP64.
+template <typename T, typename TS>
+__device__ __forceinline__ cplx<T> cvt(const cplx<TS>& v) {
+  return {T(v.x), T(v.y)};
+}
+// two adjacent complex values, global FP32 storage <-> LDS (16 B accesses)
+template <typename T>
+__device__ __forceinline__ void load_pair(cplx<T>* dst, const cplx<float>* src) {
+  const float4 v = *reinterpret_cast<const float4*>(src);
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4*>(dst) = v;
+  } else {
+    dst[0] = {T(v.x), T(v.y)};
+    dst[1] = {T(v.z), T(v.w)};
+  }
+}
+template <typename T>
+__device__ __forceinline__ void store_pair(cplx<float>* dst, const cplx<T>* src) {
+  if constexpr (sizeof(T) == 4) {
+    *reinterpret_cast<float4*>(dst) = *reinterpret_cast<const float4*>(src);
+  } else {
+    *reinterpret_cast<float4*>(dst) =
+        make_float4(float(src[0].x), float(src[0].y), float(src[1].x), float(src[1].y));
+  }
+}
+
 // ----------------------------------------------------------------- z forward
-template <typename T, int MODE, typename TIn, bool BIG>
+template <typename T, typename TS, int MODE, typename TIn, bool BIG>
 __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan_g, const cplx<T>* twz,
-                                                     int P, const TIn* in, const T* tangent,
+                                                     int P, const TIn* in, const TS* tangent,
                                                      Vec6 a, double sgn,
-                                                     const MinresState* st, cplx<T>* spec) {
+                                                     const MinresState* st, cplx<TS>* spec) {
   if (MODE == kModeMatvec && st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
@@ -99,7 +126,7 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan
         for (int i2 = 0; i2 < 6; ++i2) {
           T acc = T(0);
 #pragma unroll
-          for (int j = 0; j < 6; ++j) acc += tangent[sym6(i2, j) * g.n + p] * v[j];
+          for (int j = 0; j < 6; ++j) acc += T(tangent[sym6(i2, j) * g.n + p]) * v[j];
           tau[i2] = acc;
         }
       }
@@ -120,16 +147,16 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan
     const cplx<T> zr = cconj(res[((nz - kz) % nz) * S + m]);
     const cplx<T> xa = {half * (zf.x + zr.x), half * (zf.y + zr.y)};
     const cplx<T> xb = {half * (zf.y - zr.y), -half * (zf.x - zr.x)};
-    spec[(2 * q) * g.nspec + gl * nzc + kz] = xa;
-    spec[(2 * q + 1) * g.nspec + gl * nzc + kz] = xb;
+    spec[(2 * q) * g.nspec + gl * nzc + kz] = cvt<TS>(xa);
+    spec[(2 * q + 1) * g.nspec + gl * nzc + kz] = cvt<TS>(xb);
   }
 }
 
 // ----------------------------------------------------------------- y pass
-template <typename T, bool BIG>
+template <typename T, typename TS, bool BIG>
 __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g, const cplx<T>* twy, int TY,
                                                   int ntiles, bool inv,
-                                                  const MinresState* st, cplx<T>* spec) {
+                                                  const MinresState* st, cplx<TS>* spec) {
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
@@ -141,36 +168,40 @@ __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g,
   const int kz0 = tile * TY;
   const int width = min(TY, g.nzc - kz0);
   const int ny = g.ny;
-  cplx<T>* base = spec + (long long)cx * ny * g.nzc + kz0;  // spec[c][x][.][kz0]
+  cplx<TS>* base = spec + (long long)cx * ny * g.nzc + kz0;  // spec[c][x][.][kz0]
   load_twiddles(tw, twy, ny);
   // FP32 tiles of whole, even-aligned columns move two complex values (16 B)
   // per access (even ny keeps the LDS tile after the ny twiddles 16 B aligned)
-  const bool pairs = sizeof(T) == 4 && TY % 2 == 0 && width == TY && g.nzc % 2 == 0 && ny % 2 == 0;
+  const bool pairs = sizeof(TS) == 4 && TY % 2 == 0 && width == TY && g.nzc % 2 == 0 && ny % 2 == 0;
   const int hp = TY / 2;
-  if (pairs) {
-    for (int i = threadIdx.x; i < hp * ny; i += kYBT) {
-      const int y = i / hp, t = 2 * (i - y * hp);
-      *reinterpret_cast<float4*>(buf + y * TY + t) =
-          *reinterpret_cast<const float4*>(base + (long long)y * g.nzc + t);
+  if constexpr (sizeof(TS) == 4) {
+    if (pairs) {
+      for (int i = threadIdx.x; i < hp * ny; i += kYBT) {
+        const int y = i / hp, t = 2 * (i - y * hp);
+        load_pair<T>(buf + y * TY + t, base + (long long)y * g.nzc + t);
+      }
     }
-  } else {
+  }
+  if (!pairs) {
     for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
       const int y = i / TY, t = i - y * TY;
-      buf[y * TY + t] = t < width ? base[(long long)y * g.nzc + t] : cplx<T>{T(0), T(0)};
+      buf[y * TY + t] = t < width ? cvt<T>(base[(long long)y * g.nzc + t]) : cplx<T>{T(0), T(0)};
     }
   }
   __syncthreads();
   const cplx<T>* res = block_fft<T, kYBT, BIG>(buf, buf + TY * ny, TY, TY, plan, tw, inv);
-  if (pairs) {
-    for (int i = threadIdx.x; i < hp * ny; i += kYBT) {
-      const int y = i / hp, t = 2 * (i - y * hp);
-      *reinterpret_cast<float4*>(base + (long long)y * g.nzc + t) =
-          *reinterpret_cast<const float4*>(res + y * TY + t);
+  if constexpr (sizeof(TS) == 4) {
+    if (pairs) {
+      for (int i = threadIdx.x; i < hp * ny; i += kYBT) {
+        const int y = i / hp, t = 2 * (i - y * hp);
+        store_pair<T>(base + (long long)y * g.nzc + t, res + y * TY + t);
+      }
     }
-  } else {
+  }
+  if (!pairs) {
     for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
       const int y = i / TY, t = i - y * TY;
-      if (t < width) base[(long long)y * g.nzc + t] = res[y * TY + t];
+      if (t < width) base[(long long)y * g.nzc + t] = cvt<TS>(res[y * TY + t]);
     }
   }
 }
@@ -212,11 +243,11 @@ __device__ __forceinline__ double rnorm3<double>(double a, double b, double c) {
 // LDS: forward x-FFT of the 6 TX interleaved lines, Ghat on each point, inverse
 // x-FFT, one HBM read and one write of the tile.  Used where TX >= 2 fits the
 // budget (grids up to about 128 points along x).
-template <typename T, bool BIG>
+template <typename T, typename TS, bool BIG>
 __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
                                                   const T* tanz, int maskbits,
-                                                  const MinresState* st, cplx<T>* spec) {
+                                                  const MinresState* st, cplx<TS>* spec) {
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
@@ -229,28 +260,30 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   const int width = min(TX, g.nzc - kz0);
   const int nx = g.nx;
   const long long xstride = (long long)g.ny * g.nzc;
-  cplx<T>* base = spec + (long long)y * g.nzc + kz0;  // spec[c][0][y][kz0]
+  cplx<TS>* base = spec + (long long)y * g.nzc + kz0;  // spec[c][0][y][kz0]
   load_twiddles(tw, twx, nx);
   const int tot = 6 * nx * TX;
   // FP32 tiles of whole, even-aligned columns move two complex values (16 B)
   // per access (even nx keeps the LDS tile after the nx twiddles 16 B aligned)
-  const bool pairs = sizeof(T) == 4 && TX % 2 == 0 && width == TX && g.nzc % 2 == 0 && nx % 2 == 0;
+  const bool pairs = sizeof(TS) == 4 && TX % 2 == 0 && width == TX && g.nzc % 2 == 0 && nx % 2 == 0;
   const int hp = TX / 2;
-  if (pairs) {
-    for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
-      const int cxp = i / hp;  // c*nx + x
-      const int t = 2 * (i - cxp * hp);
-      const int c = cxp / nx, x = cxp - c * nx;
-      *reinterpret_cast<float4*>(buf + x * (6 * TX) + c * TX + t) =
-          *reinterpret_cast<const float4*>(base + c * g.nspec + x * xstride + t);
+  if constexpr (sizeof(TS) == 4) {
+    if (pairs) {
+      for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
+        const int cxp = i / hp;  // c*nx + x
+        const int t = 2 * (i - cxp * hp);
+        const int c = cxp / nx, x = cxp - c * nx;
+        load_pair<T>(buf + x * (6 * TX) + c * TX + t, base + c * g.nspec + x * xstride + t);
+      }
     }
-  } else {
+  }
+  if (!pairs) {
     for (int i = threadIdx.x; i < tot; i += kXBT) {
       const int t = i % TX;
       const int cxp = i / TX;  // c*nx + x
       const int c = cxp / nx, x = cxp - c * nx;
       buf[x * (6 * TX) + c * TX + t] =
-          t < width ? base[c * g.nspec + x * xstride + t] : cplx<T>{T(0), T(0)};
+          t < width ? cvt<T>(base[c * g.nspec + x * xstride + t]) : cplx<T>{T(0), T(0)};
     }
   }
   __syncthreads();
@@ -284,20 +317,22 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   __syncthreads();
   const cplx<T>* out =
       block_fft<T, kXBT, BIG>(res, res == buf ? buf2 : buf, 6 * TX, 6 * TX, plan, tw, true);
-  if (pairs) {
-    for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
-      const int cxp = i / hp;
-      const int t = 2 * (i - cxp * hp);
-      const int c = cxp / nx, x = cxp - c * nx;
-      *reinterpret_cast<float4*>(base + c * g.nspec + x * xstride + t) =
-          *reinterpret_cast<const float4*>(out + x * (6 * TX) + c * TX + t);
+  if constexpr (sizeof(TS) == 4) {
+    if (pairs) {
+      for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
+        const int cxp = i / hp;
+        const int t = 2 * (i - cxp * hp);
+        const int c = cxp / nx, x = cxp - c * nx;
+        store_pair<T>(base + c * g.nspec + x * xstride + t, out + x * (6 * TX) + c * TX + t);
+      }
     }
-  } else {
+  }
+  if (!pairs) {
     for (int i = threadIdx.x; i < tot; i += kXBT) {
       const int t = i % TX;
       const int cxp = i / TX;
       const int c = cxp / nx, x = cxp - c * nx;
-      if (t < width) base[c * g.nspec + x * xstride + t] = out[x * (6 * TX) + c * TX + t];
+      if (t < width) base[c * g.nspec + x * xstride + t] = cvt<TS>(out[x * (6 * TX) + c * TX + t]);
     }
   }
 }
@@ -313,12 +348,13 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
 // four workgroups per CU.
 // With PAIRS (FP32, even TX dividing nzc, even nx) a thread moves two
 // adjacent complex values (16 B) per access and the LDS rows have stride TX.
-template <typename T, bool PAIRS, bool BIG>
+template <typename T, typename TS, bool PAIRS, bool BIG>
 __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
                                                   const T* tanz, int maskbits,
-                                                  const MinresState* st, cplx<T>* spec) {
-  static_assert(!PAIRS || sizeof(T) == 4, "pairs of complex values are 16 B in FP32 only");
+                                                  const MinresState* st, cplx<TS>* spec) {
+  static_assert(!PAIRS || (sizeof(T) == 4 && sizeof(TS) == 4),
+                "pairs of complex values are 16 B in FP32 only");
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
@@ -333,13 +369,14 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan
   const int kz0 = tile * TX;
   const int width = min(TX, g.nzc - kz0);
   const long long xstride = (long long)g.ny * g.nzc;
-  cplx<T>* base = spec + (long long)y * g.nzc + kz0;  // spec[c][0][y][kz0]
+  cplx<TS>* base = spec + (long long)y * g.nzc + kz0;  // spec[c][0][y][kz0]
   load_twiddles(tw, twx, nx);
   // per-thread units of one component tile (a complex value, or a pair of
   // adjacent ones with PAIRS): (x, t) with t fastest
   constexpr int W = PAIRS ? 2 : 1;        // complex values per unit
   constexpr int K = kXTileK / W;          // units per thread
-  using Unit = typename std::conditional<PAIRS, float4, cplx<T>>::type;
+  using Unit = typename std::conditional<PAIRS, float4, cplx<T>>::type;   // in LDS
+  using GUnit = typename std::conditional<PAIRS, float4, cplx<TS>>::type;  // in HBM
   const int TU = TX / W;                  // units per row
   const int E = nx * TU;
   int goff[K], loff[K];
@@ -352,25 +389,31 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan
     loff[k] = x * S + t;
     on[k] = i < E && t < width;
   }
-  Unit pre[K];
+  GUnit pre[K];
   auto fetch = [&](int c) {
-    const Unit* src = reinterpret_cast<const Unit*>(base + c * g.nspec);
+    const GUnit* src = reinterpret_cast<const GUnit*>(base + c * g.nspec);
 #pragma unroll
-    for (int k = 0; k < K; ++k) pre[k] = on[k] ? src[goff[k] / W] : Unit{};
+    for (int k = 0; k < K; ++k) pre[k] = on[k] ? src[goff[k] / W] : GUnit{};
+  };
+  auto to_lds = [](const GUnit& u) -> Unit {
+    if constexpr (PAIRS) return u; else return cvt<T>(u);
+  };
+  auto to_hbm = [](const Unit& u) -> GUnit {
+    if constexpr (PAIRS) return u; else return cvt<TS>(u);
   };
   auto transform = [&](bool inv) {
     fetch(0);
     for (int c = 0; c < 6; ++c) {
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        if (threadIdx.x + k * kXBT < E) reinterpret_cast<Unit*>(buf)[loff[k] / W] = pre[k];
+        if (threadIdx.x + k * kXBT < E) reinterpret_cast<Unit*>(buf)[loff[k] / W] = to_lds(pre[k]);
       lds_barrier();
       if (c < 5) fetch(c + 1);
       const cplx<T>* res = block_fft<T, kXBT, BIG>(buf, buf2, TX, S, plan, tw, inv);
-      Unit* dst = reinterpret_cast<Unit*>(base + c * g.nspec);
+      GUnit* dst = reinterpret_cast<GUnit*>(base + c * g.nspec);
 #pragma unroll
       for (int k = 0; k < K; ++k)
-        if (on[k]) dst[goff[k] / W] = reinterpret_cast<const Unit*>(res)[loff[k] / W];
+        if (on[k]) dst[goff[k] / W] = to_hbm(reinterpret_cast<const Unit*>(res)[loff[k] / W]);
       lds_barrier();
     }
   };
@@ -384,10 +427,10 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan
     const int t = i % TX, kx = i / TX;
     const int kz = kz0 + t;
     if (t >= width) continue;
-    cplx<T>* pt = base + kx * xstride + t;
+    cplx<TS>* pt = base + kx * xstride + t;
     cplx<T> X[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) X[c] = pt[c * g.nspec];
+    for (int c = 0; c < 6; ++c) X[c] = cvt<T>(pt[c * g.nspec]);
     if (kx == nyqx || y == nyqy || kz == nyqz) {
 #pragma unroll
       for (int c = 0; c < 6; ++c) X[c] = {T(0), T(0)};
@@ -400,17 +443,17 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan
       apply_ghat<T>(X, tx * inv, ty * inv, tz * inv);
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) pt[c * g.nspec] = X[c];
+    for (int c = 0; c < 6; ++c) pt[c * g.nspec] = cvt<TS>(X[c]);
   }
   __syncthreads();
   transform(true);
 }
 
 // ----------------------------------------------------------------- z inverse
-template <typename T, int MODE, typename TOut, bool BIG>
+template <typename T, typename TS, int MODE, typename TOut, bool BIG>
 __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan_g, const cplx<T>* twz,
-                                                     int P, const cplx<T>* spec, TOut* out,
-                                                     T* r1, const T* r2,
+                                                     int P, const cplx<TS>* spec, TOut* out,
+                                                     TS* r1, const TS* r2,
                                                      const MinresState* st, double* partials) {
   if (MODE == kModeMatvec && st->done) return;
   extern __shared__ unsigned char smem_raw[];
@@ -428,20 +471,28 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan
   cplx<T>* stage = buf + zb;
   const int run = P * nzc;
   const long long valid = (nlines - line0 < P ? nlines - line0 : P) * nzc;
-  if (sizeof(T) == 4 && nzc % 2 == 0 && nz % 2 == 0) {
-    // FP32, even nzc: two adjacent values per access (16 B; even nz keeps the
-    // LDS stage 16 B aligned)
-    const int hr = run / 2;
-    for (int i = threadIdx.x; i < 6 * hr; i += kZBT) {
-      const int c = i / hr, r = 2 * (i - c * hr);
-      const float4 v = r < valid ? *reinterpret_cast<const float4*>(spec + c * g.nspec + line0 * nzc + r)
-                                 : make_float4(0.f, 0.f, 0.f, 0.f);
-      *reinterpret_cast<float4*>(stage + c * run + r) = v;
+  bool staged = false;
+  if constexpr (sizeof(TS) == 4) {
+    if (nzc % 2 == 0 && nz % 2 == 0) {
+      // FP32 storage, even nzc: two adjacent values per access (16 B; even nz
+      // keeps the LDS stage 16 B aligned)
+      const int hr = run / 2;
+      for (int i = threadIdx.x; i < 6 * hr; i += kZBT) {
+        const int c = i / hr, r = 2 * (i - c * hr);
+        if (r < valid) {
+          load_pair<T>(stage + c * run + r, spec + c * g.nspec + line0 * nzc + r);
+        } else {
+          stage[c * run + r] = {T(0), T(0)};
+          stage[c * run + r + 1] = {T(0), T(0)};
+        }
+      }
+      staged = true;
     }
-  } else {
+  }
+  if (!staged) {
     for (int i = threadIdx.x; i < 6 * run; i += kZBT) {
       const int c = i / run, r = i - c * run;
-      stage[i] = r < valid ? spec[c * g.nspec + line0 * nzc + r] : cplx<T>{T(0), T(0)};
+      stage[i] = r < valid ? cvt<T>(spec[c * g.nspec + line0 * nzc + r]) : cplx<T>{T(0), T(0)};
     }
   }
   __syncthreads();
@@ -488,11 +539,11 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan
         out[(2 * q + 1) * g.n + p] = TOut(vb);
       } else {
         const long long ia = (2 * q) * g.n + p, ib = (2 * q + 1) * g.n + p;
-        const T ua = va - c1 * r1[ia];
-        const T ub = vb - c1 * r1[ib];
+        const TS ua = TS(va - c1 * T(r1[ia]));
+        const TS ub = TS(vb - c1 * T(r1[ib]));
         r1[ia] = ua;
         r1[ib] = ub;
-        acc += double(s * r2[ia]) * double(ua) + double(s * r2[ib]) * double(ub);
+        acc += double(s * T(r2[ia])) * double(ua) + double(s * T(r2[ib])) * double(ub);
       }
     }
   }
@@ -554,8 +605,8 @@ void allow_lds(K kernel, size_t bytes) {
 
 }  // namespace
 
-template <typename T>
-Spectral<T>::Spectral(int nx, int ny, int nz) {
+template <typename T, typename A>
+Spectral<T, A>::Spectral(int nx, int ny, int nz) {
   if (nx < 1 || ny < 1 || nz < 1) throw Error("grid dimensions must be positive");
   g_.nx = nx;
   g_.ny = ny;
@@ -569,8 +620,8 @@ Spectral<T>::Spectral(int nx, int ny, int nz) {
   big_x_ = uses_big_radix(px_);
   big_y_ = uses_big_radix(py_);
   big_z_ = uses_big_radix(pz_);
-  const size_t cb = sizeof(cplx<T>);
-  const size_t scale = sizeof(T) / sizeof(float);
+  const size_t cb = sizeof(cplx<A>);  // LDS element
+  const size_t scale = sizeof(A) / sizeof(float);
   const size_t zbudget = kZBufBytes * scale, ybudget = kYBufBytes * scale,
                xbudget = kXBufBytes * scale;
   // z passes: P grid lines (3P interleaved complex lines) per workgroup
@@ -603,67 +654,67 @@ Spectral<T>::Spectral(int nx, int ny, int nz) {
     if (tx_ == 0) throw Error("nx too large for the x pass");
     tx_ = std::min(tx_, g_.nzc);
     // FP32 with whole, even-width tiles: pairs of complex values per access
-    x_pairs_ = sizeof(T) == 4 && tx_ % 2 == 0 && g_.nzc % tx_ == 0 && nx % 2 == 0;
+    x_pairs_ = sizeof(T) == 4 && sizeof(A) == 4 && tx_ % 2 == 0 && g_.nzc % tx_ == 0 && nx % 2 == 0;
   }
   MTL_HIP(hipMalloc(&spec_, 6 * g_.nspec * sizeof(cplx<T>)));
-  twx_ = upload(twiddle_table<T>(nx));
-  twy_ = upload(twiddle_table<T>(ny));
-  twz_ = upload(twiddle_table<T>(nz));
+  twx_ = upload(twiddle_table<A>(nx));
+  twy_ = upload(twiddle_table<A>(ny));
+  twz_ = upload(twiddle_table<A>(nz));
   pxd_ = upload(std::vector<AxisPlan>{px_});
   pyd_ = upload(std::vector<AxisPlan>{py_});
   pzd_ = upload(std::vector<AxisPlan>{pz_});
-  tanx_ = upload(tan_table<T>(nx, nx));
-  tany_ = upload(tan_table<T>(ny, ny));
-  tanz_ = upload(tan_table<T>(nz, g_.nzc));
+  tanx_ = upload(tan_table<A>(nx, nx));
+  tany_ = upload(tan_table<A>(ny, ny));
+  tanz_ = upload(tan_table<A>(nz, g_.nzc));
 }
 
-template <typename T>
-Spectral<T>::~Spectral() {
+template <typename T, typename A>
+Spectral<T, A>::~Spectral() {
   for (void* p : {(void*)spec_, (void*)twx_, (void*)twy_, (void*)twz_, (void*)tanx_,
                   (void*)tany_, (void*)tanz_, (void*)pxd_, (void*)pyd_, (void*)pzd_})
     if (p) (void)hipFree(p);
 }
 
-template <typename T>
-int Spectral<T>::zinv_blocks() const {
+template <typename T, typename A>
+int Spectral<T, A>::zinv_blocks() const {
   return ceil_div((long long)g_.nx * g_.ny, pz_lines_);
 }
 
-template <typename T>
+template <typename T, typename A>
 template <int MODE, typename TIn, typename TOut>
-void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const Vec6& a, double sgn,
+void Spectral<T, A>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const Vec6& a, double sgn,
                       const Vec6& mask, const MinresState* st, double* partials,
                       hipStream_t s) {
   const int P = pz_lines_;
   const int zblocks = zinv_blocks();
-  const size_t zlds = (size_t(g_.nz) + 2 * size_t(zbuf_elems(P, g_.nz, g_.nzc))) * sizeof(cplx<T>);
-  auto kzf = big_z_ ? k_z_forward<T, MODE, TIn, true> : k_z_forward<T, MODE, TIn, false>;
+  const size_t zlds = (size_t(g_.nz) + 2 * size_t(zbuf_elems(P, g_.nz, g_.nzc))) * sizeof(cplx<A>);
+  auto kzf = big_z_ ? k_z_forward<A, T, MODE, TIn, true> : k_z_forward<A, T, MODE, TIn, false>;
   allow_lds(kzf, zlds);
   profiler().launch(MODE == kModeMatvec ? kKZForward : residual_id(kKZForward), kzf, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, in, tangent, a, sgn, st, spec_);
   MTL_LAUNCH_CHECK();
 
   const MinresState* stp = MODE == kModeMatvec ? st : nullptr;
   const int ytiles = ceil_div(g_.nzc, ty_);
-  const size_t ylds = (size_t(g_.ny) + 2 * size_t(ty_) * g_.ny) * sizeof(cplx<T>);
-  auto ky = big_y_ ? k_y_pass<T, true> : k_y_pass<T, false>;
+  const size_t ylds = (size_t(g_.ny) + 2 * size_t(ty_) * g_.ny) * sizeof(cplx<A>);
+  auto ky = big_y_ ? k_y_pass<A, T, true> : k_y_pass<A, T, false>;
   allow_lds(ky, ylds);
   profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, false, stp, spec_);
   MTL_LAUNCH_CHECK();
 
   const int xtiles = ceil_div(g_.nzc, tx_);
   const size_t xlds =
-      x_resident_ ? (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<T>)
-      : x_pairs_  ? (size_t(g_.nx) + 2 * size_t(tx_) * g_.nx) * sizeof(cplx<T>)
-                  : x_lds_bytes(g_.nx, tx_, sizeof(cplx<T>));
-  auto kx = big_x_ ? k_x_ghat_resident<T, true> : k_x_ghat_resident<T, false>;
+      x_resident_ ? (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<A>)
+      : x_pairs_  ? (size_t(g_.nx) + 2 * size_t(tx_) * g_.nx) * sizeof(cplx<A>)
+                  : x_lds_bytes(g_.nx, tx_, sizeof(cplx<A>));
+  auto kx = big_x_ ? k_x_ghat_resident<A, T, true> : k_x_ghat_resident<A, T, false>;
   if (!x_resident_) {
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4 && sizeof(A) == 4) {
       if (x_pairs_)
-        kx = big_x_ ? k_x_ghat_streamed<T, true, true> : k_x_ghat_streamed<T, true, false>;
+        kx = big_x_ ? k_x_ghat_streamed<A, T, true, true> : k_x_ghat_streamed<A, T, true, false>;
       else
-        kx = big_x_ ? k_x_ghat_streamed<T, false, true> : k_x_ghat_streamed<T, false, false>;
+        kx = big_x_ ? k_x_ghat_streamed<A, T, false, true> : k_x_ghat_streamed<A, T, false, false>;
     } else {
-      kx = big_x_ ? k_x_ghat_streamed<T, false, true> : k_x_ghat_streamed<T, false, false>;
+      kx = big_x_ ? k_x_ghat_streamed<A, T, false, true> : k_x_ghat_streamed<A, T, false, false>;
     }
   }
   allow_lds(kx, xlds);
@@ -673,21 +724,21 @@ void Spectral<T>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const V
   profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, true, stp, spec_);
   MTL_LAUNCH_CHECK();
 
-  auto kzi = big_z_ ? k_z_inverse<T, MODE, TOut, true> : k_z_inverse<T, MODE, TOut, false>;
+  auto kzi = big_z_ ? k_z_inverse<A, T, MODE, TOut, true> : k_z_inverse<A, T, MODE, TOut, false>;
   allow_lds(kzi, zlds);
   profiler().launch(MODE == kModeMatvec ? kKZInverse : residual_id(kKZInverse), kzi, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, spec_, out, r1, reinterpret_cast<const T*>(in), st, partials);
   MTL_LAUNCH_CHECK();
 }
 
-template <typename T>
+template <typename T, typename A>
 template <typename TIn, typename TOut>
-void Spectral<T>::project(const TIn* in, TOut* out, const Vec6& a, double sgn, const Vec6& mask,
+void Spectral<T, A>::project(const TIn* in, TOut* out, const Vec6& a, double sgn, const Vec6& mask,
                           hipStream_t s) {
   run<kModeProject, TIn, TOut>(in, nullptr, out, nullptr, a, sgn, mask, nullptr, nullptr, s);
 }
 
-template <typename T>
-int Spectral<T>::matvec_minres(const T* r2, const T* tangent, T* r1, const Vec6& mask,
+template <typename T, typename A>
+int Spectral<T, A>::matvec_minres(const T* r2, const T* tangent, T* r1, const Vec6& mask,
                                const MinresState* st, double* partials, hipStream_t s) {
   Vec6 zero{};
   run<kModeMatvec, T, T>(r2, tangent, static_cast<T*>(nullptr), r1, zero, 1.0, mask, st,
@@ -697,11 +748,15 @@ int Spectral<T>::matvec_minres(const T* r2, const T* tangent, T* r1, const Vec6&
 
 template class Spectral<float>;
 template class Spectral<double>;
+template class Spectral<float, double>;
 template void Spectral<double>::project<double, double>(const double*, double*, const Vec6&,
                                                         double, const Vec6&, hipStream_t);
 template void Spectral<float>::project<double, float>(const double*, float*, const Vec6&, double,
                                                       const Vec6&, hipStream_t);
 template void Spectral<float>::project<double, double>(const double*, double*, const Vec6&,
                                                        double, const Vec6&, hipStream_t);
+template void Spectral<float, double>::project<double, double>(const double*, double*,
+                                                               const Vec6&, double, const Vec6&,
+                                                               hipStream_t);
 
 }  // namespace mtl

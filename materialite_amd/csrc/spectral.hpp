@@ -16,7 +16,10 @@ __host__ __device__ constexpr int sym6(int i, int j) {
   return i <= j ? i * 6 - (i * (i - 1)) / 2 + (j - i) : j * 6 - (j * (j - 1)) / 2 + (i - j);
 }
 
-template <typename T>
+// T: type of the spectrum and of the Krylov vectors and tangent in HBM;
+// A: arithmetic type of the FFT passes (LDS tiles, twiddles, Ghat).  A = double
+// with T = float transforms FP32 data with FP64 butterflies.
+template <typename T, typename A = T>
 class Spectral {
  public:
   Spectral(int nx, int ny, int nz);
@@ -55,12 +58,12 @@ class Spectral {
   bool x_pairs_ = false;  // streamed x pass moves pairs of complex values
   bool big_x_ = false, big_y_ = false, big_z_ = false;  // axis plans use radix 11 or 13
   cplx<T>* spec_ = nullptr;
-  cplx<T>* twx_ = nullptr;
-  cplx<T>* twy_ = nullptr;
-  cplx<T>* twz_ = nullptr;
-  T* tanx_ = nullptr;
-  T* tany_ = nullptr;
-  T* tanz_ = nullptr;
+  cplx<A>* twx_ = nullptr;
+  cplx<A>* twy_ = nullptr;
+  cplx<A>* twz_ = nullptr;
+  A* tanx_ = nullptr;
+  A* tany_ = nullptr;
+  A* tanz_ = nullptr;
 };
 
 }  // namespace mtl

@@ -8,9 +8,10 @@ runs on the device through ``mtl_solver_step``; this module keeps the
 reference's host-side control flow and the ``Material`` round trip.
 
 Extra keyword (constructor): ``precision`` = ``"fp64"`` (default; every field,
-FFT and Krylov vector in FP64, reproducing the reference) or ``"mixed"`` (FP32
+FFT and Krylov vector in FP64, reproducing the reference), ``"mixed"`` (FP32
 Krylov vectors, tangents and matvec FFTs; FP64 strain/stress, residual FFT and
-norms).
+norms) or ``"mixed-fft64"`` (as ``"mixed"``, with FP64 arithmetic inside the
+matvec FFT passes: the data stay FP32 in device memory).
 """
 
 import logging

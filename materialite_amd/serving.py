@@ -55,8 +55,9 @@ def footprint_bytes(dims, precision="mixed", evp_points=0, num_slip_systems=12):
     n = int(np.prod(dims))
     nz = int(dims[2])
     h = ((nz + 1) // 2) / nz
-    k = 4 if precision == "mixed" else 8
-    per_voxel = _FIELDS + 6 * 16 * h + (6 * 8 * h if precision == "mixed" else 0)
+    mixed = precision in ("mixed", "mixed-fft64")
+    k = 4 if mixed else 8
+    per_voxel = _FIELDS + 6 * 16 * h + (6 * 8 * h if mixed else 0)
     per_voxel += 6 * 6 * k + 2 * 21 * k + _ELASTIC_POINT
     evp = evp_points * (_EVP_POINT_FIXED + num_slip_systems * _EVP_POINT_PER_SYSTEM)
     # allocation granularity (about 30 device allocations per handle) and

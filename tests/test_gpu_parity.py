@@ -35,7 +35,7 @@ GRIDS = [(3, 4, 5), (4, 4, 4), (2, 3, 6), (1, 4, 3), (6, 5, 8), (8, 8, 8), (7, 9
 
 
 @pytest.mark.parametrize("dims", GRIDS)
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-fft64"])
 def test_projection_matches_oracle(dims, precision):
     from materialite_amd._native import NativeSolver
     from oracle.projection import apply_projection
@@ -88,6 +88,7 @@ def test_invalid_model_arguments_are_errors():
 
 
 @pytest.mark.parametrize("dims,precision", [((128, 128, 128), "fp64"), ((128, 128, 128), "mixed"),
+                                             ((128, 128, 128), "mixed-fft64"),
                                              ((256, 256, 256), "mixed")])
 def test_projection_properties_full_size(dims, precision):
     """At the workload sizes (128^3, and 256^3 where the x pass streams the
@@ -132,7 +133,7 @@ def _pore_case():
     return rot, pore, mp
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-fft64"])
 def test_minres_matches_oracle(precision):
     from materialite_amd._native import NativeSolver
     from oracle.minres import minres
@@ -172,7 +173,7 @@ def test_minres_matches_oracle(precision):
         assert np.abs(x - x_ref).max() <= 1e-3 * scale
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-fft64"])
 @pytest.mark.parametrize("rate_exponent", [10.0, 10.5])
 def test_evp_update_matches_oracle(precision, rate_exponent):
     """rate exponent 10: |tau/g|^9 by squaring; 10.5: the general pow()."""

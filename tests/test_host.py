@@ -140,3 +140,10 @@ def test_hcp_slip_systems_match_oracle():
                     (7 + 5 * np.sqrt(3)) / (12 * np.sqrt(13)))
     assert_allclose(SlipSystem.octahedral().max_schmid_factor(Vector([1.0, 0.0, 0.0])).components,
                     1 / np.sqrt(6))
+
+
+def test_unknown_precision_is_rejected():
+    from materialite_amd._native import NativeSolver
+
+    with pytest.raises(ValueError, match="precision must be one of"):
+        NativeSolver((4, 4, 4), "fp16")
