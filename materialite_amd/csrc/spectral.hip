@@ -32,7 +32,7 @@ constexpr int kXTileK = 8;
 // LDS bytes of the x pass's two single-component buffers (doubled for FP64)
 constexpr size_t kXPairBytes = 40 * 1024;
 // LDS bytes per FP32 workgroup for the two ping-pong line buffers (doubled for
-// FP64).  Small tiles keep many workgroups resident per CU (a CU has 160 KB),
+// FP64 arithmetic, whatever the storage type).  Small tiles keep many workgroups resident per CU (a CU has 160 KB),
 // which hides the HBM latency of the strided tile loads: measured at 128^3,
 // these sizes beat 32-96 KB tiles by 25-40 %.
 constexpr size_t kZBufBytes = 16 * 1024, kYBufBytes = 16 * 1024, kXBufBytes = 24 * 1024;
@@ -170,8 +170,9 @@ __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g,
   const int ny = g.ny;
   cplx<TS>* base = spec + (long long)cx * ny * g.nzc + kz0;  // spec[c][x][.][kz0]
   load_twiddles(tw, twy, ny);
-  // FP32 tiles of whole, even-aligned columns move two complex values (16 B)
-  // per access (even ny keeps the LDS tile after the ny twiddles 16 B aligned)
+  // FP32-stored tiles of whole, even-aligned columns move two complex values
+  // (16 B) per access (even ny keeps the LDS tile after the ny twiddles 16 B
+  // aligned)
   const bool pairs = sizeof(TS) == 4 && TY % 2 == 0 && width == TY && g.nzc % 2 == 0 && ny % 2 == 0;
   const int hp = TY / 2;
   if constexpr (sizeof(TS) == 4) {
@@ -263,8 +264,9 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   cplx<TS>* base = spec + (long long)y * g.nzc + kz0;  // spec[c][0][y][kz0]
   load_twiddles(tw, twx, nx);
   const int tot = 6 * nx * TX;
-  // FP32 tiles of whole, even-aligned columns move two complex values (16 B)
-  // per access (even nx keeps the LDS tile after the nx twiddles 16 B aligned)
+  // FP32-stored tiles of whole, even-aligned columns move two complex values
+  // (16 B) per access (even nx keeps the LDS tile after the nx twiddles 16 B
+  // aligned)
   const bool pairs = sizeof(TS) == 4 && TX % 2 == 0 && width == TX && g.nzc % 2 == 0 && nx % 2 == 0;
   const int hp = TX / 2;
   if constexpr (sizeof(TS) == 4) {
