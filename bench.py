@@ -77,6 +77,10 @@ def parse():
                          "dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-dispatch HBM traffic summary of two rocprofv3 --pmc runs")
+    ap.add_argument("--stream-only", action="store_true",
+                    help="run only the serving stream and print its summary (for tracing it)")
+    ap.add_argument("--summarize-stream-trace", metavar="KERNEL_TRACE_CSV",
+                    help="print the device occupancy of a kernel trace of --stream-only")
     ap.add_argument("--summarize-trace", metavar="KERNEL_TRACE_CSV",
                     help="print the per-kernel table of a rocprofv3 --kernel-trace run")
     ap.add_argument("--config1", type=int, default=1,
@@ -448,6 +452,41 @@ def print_trace_summary(trace_csv):
               f"{sum(w) / len(w):.1f} | {100 * sum(v) / total:.1f} |")
 
 
+def print_stream_trace_summary(trace_csv):
+    """Device occupancy of a kernel trace of the serving stream alone
+    (--stream-only): how much of the span some kernel runs, how much of it two
+    or more kernels from different request streams run at once, and the
+    largest number of kernels in flight together."""
+    import csv
+
+    rows = list(csv.DictReader(open(trace_csv)))
+    ev = []
+    queues = set()
+    for r in rows:
+        a, b = int(r["Start_Timestamp"]), int(r["End_Timestamp"])
+        ev.append((a, 1))
+        ev.append((b, -1))
+        queues.add((r.get("Queue_Id"), r.get("Stream_Id")))
+    ev.sort()
+    span = ev[-1][0] - ev[0][0]
+    busy = multi = 0
+    depth = peak = 0
+    last = ev[0][0]
+    for t, d in ev:
+        if depth >= 1:
+            busy += t - last
+        if depth >= 2:
+            multi += t - last
+        depth += d
+        peak = max(peak, depth)
+        last = t
+    print(f"dispatches: {len(rows)} on {len(queues)} (queue, stream) pairs; span {span / 1e9:.3f} s")
+    print(f"some kernel running: {100 * busy / span:.1f} % of the span")
+    print(f"two or more kernels running at once: {100 * multi / span:.1f} % of the span "
+          f"({100 * multi / max(busy, 1):.1f} % of the busy time)")
+    print(f"most kernels in flight together: {peak}")
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     (scaled FETCH + WRITE), or None."""
@@ -650,6 +689,9 @@ def main():
     if args.oracle_child:
         oracle_child(*args.oracle_child)
         return
+    if args.summarize_stream_trace:
+        print_stream_trace_summary(args.summarize_stream_trace)
+        return
     if args.summarize_trace:
         print_trace_summary(args.summarize_trace)
         return
@@ -664,6 +706,10 @@ def main():
     import torch
 
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    if args.stream_only:
+        print(json.dumps({"serving": serving_stream(args.stream_requests, args.stream_rate,
+                                                    args.stream_slots)}))
+        return
     if args.copy_probe:
         src = torch.ones(args.copy_probe * 2 ** 18, dtype=torch.float32, device="cuda")
         dst = torch.empty_like(src)
