@@ -44,11 +44,15 @@ def slip_systems(families):
     return ss
 
 
-@pytest.mark.parametrize("name", list(FAMILIES))
-def test_hcp_evp_fields_match_oracle(name):
+@pytest.mark.parametrize("name,precision",
+                         [(f, "fp64") for f in FAMILIES]
+                         + [(f, p) for f in ("basal_prismatic", "all") for p in ("mixed", "mixed-fft64")])
+def test_hcp_evp_fields_match_oracle(name, precision):
+    """FP64: fields equal the oracle's to 1e-9.  Mixed forms: stress within the
+    1e-5 field tolerance and slip resistances within 1e-4 of the oracle."""
     fams = FAMILIES[name]
     ss = slip_systems(fams)
-    print(f"hcp {name}: {len(ss)} systems", flush=True)
+    print(f"hcp {name} {precision}: {len(ss)} systems", flush=True)
     rng = np.random.default_rng(7)
     material = (Material(dimensions=[8, 8, 8], sizes=[3, 3, 3])
                 .create_voronoi(num_regions=10, rng=rng)
@@ -59,16 +63,17 @@ def test_hcp_evp_fields_match_oracle(name):
                                 hardening_properties=VOCE)
     ls = LoadSchedule.from_constant_uniaxial_strain_rate(direction="z")
     out = SmallStrainFFT(load_schedule=ls, end_time=4e-3, initial_time_increment=1e-3,
-                         constitutive_model=model)(material, global_tolerance=1e-7,
-                                                  output_variables=["slip_resistances"])
+                         constitutive_model=model, precision=precision)(
+        material, global_tolerance=1e-7, output_variables=["slip_resistances"])
     n, d = T.hcp_slip_systems(fams)
     omodel = C.ElasticViscoplastic(T.stiffness_from_voigt(TI), T.schmid_mandel(n, d), 1.0, 10.0,
                                    200.0, "voce", VOCE)
     ref = OSolver(OLS.from_constant_uniaxial_strain_rate(direction="z"), end_time=4e-3,
                   initial_time_increment=1e-3).run([8, 8, 8], rot, omodel, global_tolerance=1e-7)
+    stol, gtol = (1e-9, 1e-9) if precision == "fp64" else (1e-5, 1e-4)
     s, so = out.extract("stress").components, ref["stress"]
-    assert np.abs(s - so).max() <= 1e-9 * np.abs(so).max()
+    assert np.abs(s - so).max() <= stol * np.abs(so).max(), np.abs(s - so).max() / np.abs(so).max()
     g, go = out.extract("slip_resistances").components, ref["slip_resistances"]
     assert g.shape == go.shape == (512, len(ss))
-    assert np.abs(g - go).max() <= 1e-9 * np.abs(go).max()
+    assert np.abs(g - go).max() <= gtol * np.abs(go).max()
     assert np.abs(go - 200.0).max() > 1.0  # the crystals hardened
