@@ -152,7 +152,9 @@ int mtl_solver_constitutive(mtl_solver* h, const double* strain_dev, const doubl
  * with Ghat, 3 z-inverse FFT pass, 4 MINRES vector updates, 5 MINRES scalar
  * steps, 6 EVP iteration, 7 EVP outputs, 8 elastic update, 9 Newton
  * reductions, 10-13 the four FFT passes (ids 0-3) when they project a
- * residual instead of applying the Krylov matvec. */
+ * residual instead of applying the Krylov matvec.  The records are shared by
+ * all handles and safe to use from several threads; while timing is on, the
+ * kernels of concurrent handles are recorded together. */
 int mtl_profile_enable(int enable);
 int mtl_profile_reset(void);
 int mtl_profile_read(int kernel, double* total_ms, long long* launches);
