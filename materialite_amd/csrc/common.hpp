@@ -108,6 +108,16 @@ __device__ __forceinline__ double block_max(double v, double* red) {
 
 inline int ceil_div(long long a, long long b) { return static_cast<int>((a + b - 1) / b); }
 
+// Tangent layout: the 21 packed upper-triangle entries of each point, blocked
+// by 64 points: entry k of point p sits at ((p / 64) * 21 + k) * 64 + p % 64.
+// A wavefront of 64 consecutive points then reads its whole tangent as one
+// contiguous 21 x 256 B run (FP32) instead of 21 streams far apart.
+__host__ __device__ inline long long tan_idx(int k, long long p) {
+  return ((p >> 6) * 21 + k) * 64 + (p & 63);
+}
+// entries to allocate for n points (the last block padded to 64)
+inline long long tan_elems(long long n) { return 21 * ((n + 63) / 64) * 64; }
+
 // Copy a small parameter block from device memory into LDS with all threads of
 // the workgroup (the caller issues the barrier).  Large uniform parameter sets
 // passed by value live in SGPRs and spill; reading them from LDS avoids that.

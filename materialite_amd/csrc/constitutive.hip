@@ -131,7 +131,7 @@ template <typename K>
 __device__ __forceinline__ void store_tangent(K* tangent, long long npts, long long p,
                                               const double Ds[21]) {
 #pragma unroll
-  for (int k = 0; k < 21; ++k) tangent[k * npts + p] = K(Ds[k]);
+  for (int k = 0; k < 21; ++k) tangent[tan_idx(k, p)] = K(Ds[k]);
 }
 
 // tangent <- Rm^T Dc Rm as tangent_to_specimen + store_tangent (the same sums in
@@ -156,7 +156,7 @@ __device__ __forceinline__ void store_tangent_to_specimen(const double Rm[36], c
       double acc = 0.0;
 #pragma unroll
       for (int i = 0; i < 6; ++i) acc += Rm[i * 6 + a] * Tb[i];
-      tangent[sym6(a, b) * npts + p] = K(acc);
+      tangent[tan_idx(sym6(a, b), p)] = K(acc);
     }
   }
 }

@@ -14,7 +14,7 @@ class Minres {
   Minres(const Minres&) = delete;
   Minres& operator=(const Minres&) = delete;
 
-  // Solve with x0 = 0; b is a double SoA [6][n] field, tangent T [21][n].
+  // Solve with x0 = 0; b is a double SoA [6][n] field, tangent T [n/64][21][64].
   // Returns the iteration count (SciPy's callback count); istop as SciPy.
   int solve(const double* b, const T* tangent, const Vec6& mask, double rtol, long long maxiter,
             hipStream_t s, int* istop_out);

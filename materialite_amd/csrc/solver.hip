@@ -4,7 +4,7 @@
 // (constitutive update, residual b = Ghat*(Sigma - sigma), MINRES correction,
 // Abaqus-style equilibrium / strain-correction tests), commit and cutback.
 // Host code keeps only scalars; every field lives in HBM as SoA [6][N] FP64
-// (strain, stress, residual) or [21][N] K (tangent, K = float or double).
+// (strain, stress, residual) or [N/64][21][64] K (tangent, K = float or double).
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -57,7 +57,7 @@ __global__ void __launch_bounds__(kNBT) k_guess(long long n, int step0, const do
       if (step0) {
         double acc = 0.0;
 #pragma unroll
-        for (int b = 0; b < 6; ++b) acc += double(D[sym6(a, b) * n + p]) * de.v[b];
+        for (int b = 0; b < 6; ++b) acc += double(D[tan_idx(sym6(a, b), p)]) * de.v[b];
         g = ds.v[a] + acc;
       } else {
         g = sig_old[a * n + p] + ds.v[a];
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(kNBT) k_newton_norms(long long n, const double
       for (int a = 0; a < 6; ++a) {
         double acc = 0.0;
 #pragma unroll
-        for (int c = 0; c < 6; ++c) acc += double(D[sym6(a, c) * n + p]) * xv[c];
+        for (int c = 0; c < 6; ++c) acc += double(D[tan_idx(sym6(a, c), p)]) * xv[c];
         q2 += acc * acc;
       }
       mdx = fmax(mdx, q2);
@@ -172,7 +172,7 @@ __global__ void __launch_bounds__(kNBT) k_tangent_sum(long long n, const K* D, d
   for (long long p = blockIdx.x * (long long)kNBT + threadIdx.x; p < n;
        p += (long long)gridDim.x * kNBT)
 #pragma unroll
-    for (int k = 0; k < 21; ++k) acc[k] += double(D[k * n + p]);
+    for (int k = 0; k < 21; ++k) acc[k] += double(D[tan_idx(k, p)]);
 #pragma unroll
   for (int k = 0; k < 21; ++k) {
     const double t = block_sum<kNBT>(acc[k], red);
@@ -267,8 +267,8 @@ class Solver final : public SolverBase {
     auto dalloc = [&](double*& p, long long c) { MTL_HIP(hipMalloc(&p, c * sizeof(double))); };
     for (double** f : {&eps_, &sig_, &sig_old_, &eps_old_, &fluct_, &guess_, &b_}) dalloc(*f, 6 * n_);
     dalloc(rot_, 9 * n_);
-    MTL_HIP(hipMalloc(&D_, 21 * n_ * sizeof(K)));
-    MTL_HIP(hipMalloc(&D_old_, 21 * n_ * sizeof(K)));
+    MTL_HIP(hipMalloc(&D_, tan_elems(n_) * sizeof(K)));
+    MTL_HIP(hipMalloc(&D_old_, tan_elems(n_) * sizeof(K)));
     dalloc(partials_, kNormBlocks * 21);
     dalloc(shift_, 6);
     MTL_HIP(hipHostMalloc(&hpart_, kNormBlocks * 21 * sizeof(double), hipHostMallocDefault));
@@ -319,9 +319,9 @@ class Solver final : public SolverBase {
     if (models_.empty()) throw Error("set the microstructure before initialize");
     for (double* f : {eps_, sig_, sig_old_, eps_old_, fluct_, guess_, b_})
       MTL_HIP(hipMemsetAsync(f, 0, 6 * n_ * sizeof(double), s_));
-    MTL_HIP(hipMemsetAsync(D_, 0, 21 * n_ * sizeof(K), s_));
+    MTL_HIP(hipMemsetAsync(D_, 0, tan_elems(n_) * sizeof(K), s_));
     for (auto& m : models_) m->init_tangent(rot_, D_, s_);
-    MTL_HIP(hipMemcpyAsync(D_old_, D_, 21 * n_ * sizeof(K), hipMemcpyDeviceToDevice, s_));
+    MTL_HIP(hipMemcpyAsync(D_old_, D_, tan_elems(n_) * sizeof(K), hipMemcpyDeviceToDevice, s_));
     sig_stale_ = eps_stale_ = d_stale_ = guess_is_stress_ = false;
     stream_sync(s_);
   }
@@ -547,11 +547,11 @@ class Solver final : public SolverBase {
       return;
     }
     if (which == kFieldTangent) {
-      std::vector<K> soa(21 * size_t(n_));
-      MTL_HIP(hipMemcpy(soa.data(), cur_D(), soa.size() * sizeof(K), hipMemcpyDeviceToHost));
+      std::vector<K> blk(tan_elems(n_));
+      MTL_HIP(hipMemcpy(blk.data(), cur_D(), blk.size() * sizeof(K), hipMemcpyDeviceToHost));
       for (long long p = 0; p < n_; ++p)
         for (int a = 0; a < 6; ++a)
-          for (int b = 0; b < 6; ++b) out[p * 36 + a * 6 + b] = double(soa[size_t(sym6(a, b)) * n_ + p]);
+          for (int b = 0; b < 6; ++b) out[p * 36 + a * 6 + b] = double(blk[tan_idx(sym6(a, b), p)]);
       *ncomp = 36;
       return;
     }
@@ -609,7 +609,7 @@ class Solver final : public SolverBase {
     MTL_LAUNCH_CHECK();
   }
   void restore_tangent() {
-    MTL_HIP(hipMemcpyAsync(D_, D_old_, 21 * n_ * sizeof(K), hipMemcpyDeviceToDevice, s_));
+    MTL_HIP(hipMemcpyAsync(D_, D_old_, tan_elems(n_) * sizeof(K), hipMemcpyDeviceToDevice, s_));
   }
   // Reorders on the device into b_ (the Newton residual, rewritten before it
   // is read in every step) and copies the point-major result to `out`.

@@ -1,6 +1,6 @@
 // FFT passes of the projection apply and of the MINRES matvec (gfx950).
 //
-// Data layout (all SoA): real fields [6][n]; tangent [21][n] packed upper
+// Data layout: real fields SoA [6][n]; tangent [n/64][21][64] packed upper
 // triangle; spectrum [6][nx][ny][nzc] complex.  One matvec is five kernels:
 //   z_forward  : D:(s r2) (or a + sgn*in), paired real->complex z-FFT
 //   y_pass     : complex y-FFT of kz tiles (forward)
@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan
         for (int i2 = 0; i2 < 6; ++i2) {
           T acc = T(0);
 #pragma unroll
-          for (int j = 0; j < 6; ++j) acc += T(tangent[sym6(i2, j) * g.n + p]) * v[j];
+          for (int j = 0; j < 6; ++j) acc += T(tangent[tan_idx(sym6(i2, j), p)]) * v[j];
           tau[i2] = acc;
         }
       }
