@@ -118,6 +118,14 @@ __host__ __device__ inline long long tan_idx(int k, long long p) {
 // entries to allocate for n points (the last block padded to 64)
 inline long long tan_elems(long long n) { return 21 * ((n + 63) / 64) * 64; }
 
+// The same 64-point blocking for a per-point array of C components (the EVP
+// state): component c of point i at ((i / 64) * C + c) * 64 + i % 64, so that
+// the components of point i sit 64 apart from blk_idx(0, i, C).
+__host__ __device__ inline long long blk_idx(int c, long long i, int C) {
+  return ((i >> 6) * C + c) * 64 + (i & 63);
+}
+inline long long blk_elems(long long n, int C) { return C * ((n + 63) / 64) * 64; }
+
 // Copy a small parameter block from device memory into LDS with all threads of
 // the workgroup (the caller issues the barrier).  Large uniform parameter sets
 // passed by value live in SGPRs and spill; reading them from LDS avoids that.

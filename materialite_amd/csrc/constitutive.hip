@@ -405,16 +405,17 @@ __device__ __forceinline__ bool evp_converged(const EvpParams<NS>& q, double dif
 }
 
 struct EvpBuffers {
-  double* e0;        // [6][n] crystal strain minus committed plastic strain
+  // 6- and NS-component arrays are blocked by 64 points (blk_idx)
+  double* e0;        // [6] crystal strain minus committed plastic strain
   double* gnorm;     // [n]
-  double* sig[2];    // [6][n] ping-pong
+  double* sig[2];    // [6] ping-pong
   double* dsl[2];    // [n] ping-pong slip increment d of the iteration
   const double* g_trial;  // [n] slip resistance of the first iteration
   unsigned char* conv;
   int* last;         // [n] last iteration each point ran
-  double* ep_old;    // [6][n]
+  double* ep_old;    // [6]
   double* g_old;     // [n] (every system of a point has the same value)
-  double* gam_old;   // [NS][n]
+  double* gam_old;   // [NS]
   double* acc_old;   // [n]
   double* ep_pend;   // outputs of the call, adopted as trial state on success
   double* g_pend;
@@ -435,8 +436,7 @@ __global__ void __launch_bounds__(kCBT) k_evp_setup(int n, const int* idx, long 
   for (int c = 0; c < 6; ++c) v[c] = eps[c * npts + p];
   to_crystal(Rm, v, vc);
 #pragma unroll
-  for (int c = 0; c < 6; ++c)
-    B.e0[c * (long long)n + i] = vc[c] - B.ep_old[c * (long long)n + i];
+  for (int c = 0; c < 6; ++c) B.e0[blk_idx(c, i, 6)] = vc[c] - B.ep_old[blk_idx(c, i, 6)];
   double nrm = 0.0;
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
@@ -446,7 +446,7 @@ __global__ void __launch_bounds__(kCBT) k_evp_setup(int n, const int* idx, long 
   B.gnorm[i] = sqrt(nrm);
   to_crystal(Rm, v, vc);
 #pragma unroll
-  for (int c = 0; c < 6; ++c) B.sig[0][c * (long long)n + i] = vc[c];
+  for (int c = 0; c < 6; ++c) B.sig[0][blk_idx(c, i, 6)] = vc[c];
 }
 
 __global__ void k_evp_ctrl_reset(EvpControl* c) {
@@ -460,10 +460,9 @@ __global__ void k_evp_ctrl_reset(EvpControl* c) {
 template <int NS>
 __device__ __forceinline__ void load_point(const EvpParams<NS>& q, const EvpBuffers& B, int n,
                                            int i, int it, double sig[6], double& g) {
-  const long long ln = n;
   const int buf = (it - 1) & 1;
 #pragma unroll
-  for (int c = 0; c < 6; ++c) sig[c] = B.sig[buf][c * ln + i];
+  for (int c = 0; c < 6; ++c) sig[c] = B.sig[buf][blk_idx(c, i, 6)];
   g = it == 1 ? B.g_trial[i] : evp_harden<NS>(q, B.g_old[i], B.acc_old[i], B.dsl[buf][i]);
 }
 
@@ -493,13 +492,13 @@ __global__ void __launch_bounds__(kCBT, evp_iter_min_blocks(NS, IP)) k_evp_iter(
   __syncthreads();
   if (active) {
     const int dst = it & 1;
-    const long long ln = n;
     double sig[6], g;
     load_point<NS>(q, B, n, i, it, sig, g);
     double sig_new[6], ep[6], S[21], diff, d;
-    evp_iterate<NS, IP>(q, sig, g, B.e0 + i, nullptr, ln, sig_new, ep, S, nullptr, diff, d);
+    evp_iterate<NS, IP>(q, sig, g, B.e0 + blk_idx(0, i, 6), nullptr, 64, sig_new, ep, S, nullptr,
+                        diff, d);
 #pragma unroll
-    for (int c = 0; c < 6; ++c) B.sig[dst][c * ln + i] = sig_new[c];
+    for (int c = 0; c < 6; ++c) B.sig[dst][blk_idx(c, i, 6)] = sig_new[c];
     B.dsl[dst][i] = d;
     const bool cv = evp_converged<NS>(q, diff, B.gnorm[i]);
     B.conv[i] = cv ? 1 : 0;
@@ -547,23 +546,23 @@ __global__ void __launch_bounds__(kCBT) k_evp_outputs(int n, const int* idx, lon
   __syncthreads();
   const int i = blockIdx.x * kCBT + threadIdx.x;
   if (i >= n) return;
-  const long long ln = n;
   double sig[6];
   double sig_new[6], ep[6], S[21], Sinv[21], diff, d;
   double g;
   double* dgam = dgsh + threadIdx.x;
   load_point<NS>(q, B, n, i, B.last[i], sig, g);
-  evp_iterate<NS, IP>(q, sig, g, B.e0 + i, B.ep_old + i, ln, sig_new, ep, S, dgam, diff, d);
+  evp_iterate<NS, IP>(q, sig, g, B.e0 + blk_idx(0, i, 6), B.ep_old + blk_idx(0, i, 6), 64,
+                      sig_new, ep, S, dgam, diff, d);
   // trial state first (frees its registers before the rotations)
   const double acc_old = B.acc_old[i];
   B.g_pend[i] = evp_harden<NS>(q, B.g_old[i], acc_old, d);
 #pragma unroll
-  for (int c = 0; c < 6; ++c) B.ep_pend[c * ln + i] = ep[c];
+  for (int c = 0; c < 6; ++c) B.ep_pend[blk_idx(c, i, 6)] = ep[c];
   double asum = 0.0;
 #pragma unroll 1
   for (int s = 0; s < NS; ++s) {
     const double dg = dgam[s * kCBT];
-    B.gam_pend[s * ln + i] = B.gam_old[s * ln + i] + dg;
+    B.gam_pend[blk_idx(s, i, NS)] = B.gam_old[blk_idx(s, i, NS)] + dg;
     asum += fabs(dg);
   }
   B.acc_pend[i] = acc_old + asum;
@@ -587,7 +586,7 @@ __global__ void __launch_bounds__(kCBT) k_ep_specimen(int n, const int* idx, lon
   double Rm[36], vc[6], vs[6];
   load_rotation(rot9, npts, idx[i], Rm);
 #pragma unroll
-  for (int c = 0; c < 6; ++c) vc[c] = ep[c * (long long)n + i];
+  for (int c = 0; c < 6; ++c) vc[c] = ep[blk_idx(c, i, 6)];
   to_specimen(Rm, vc, vs);
 #pragma unroll
   for (int c = 0; c < 6; ++c) out[i * 6LL + c] = vs[c];
@@ -720,18 +719,18 @@ class EvpModel final : public Model<K> {
     for (int k = 0; k < 4; ++k) q_.hp[k] = d.hp[k];
     const long long n = std::max(this->n_, 1);
     auto alloc = [&](double*& ptr, long long cnt) { MTL_HIP(hipMalloc(&ptr, cnt * sizeof(double))); };
-    alloc(B_.e0, 6 * n);
+    alloc(B_.e0, blk_elems(n, 6));
     alloc(B_.gnorm, n);
     for (int k = 0; k < 2; ++k) {
-      alloc(B_.sig[k], 6 * n);
+      alloc(B_.sig[k], blk_elems(n, 6));
       alloc(B_.dsl[k], n);
     }
     MTL_HIP(hipMalloc(&B_.conv, n));
     MTL_HIP(hipMalloc(&B_.last, n * sizeof(int)));
     for (auto& st : slot_) {
-      alloc(st.ep, 6 * n);
+      alloc(st.ep, blk_elems(n, 6));
       alloc(st.g, n);
-      alloc(st.gam, NS * n);
+      alloc(st.gam, blk_elems(n, NS));
       alloc(st.acc, n);
     }
     bind();
@@ -762,9 +761,9 @@ class EvpModel final : public Model<K> {
     ring_ = SlotRing{};
     bind();
     const EvpState& st = slot_[ring_.old];  // committed and trial state
-    fill(st.ep, 6 * n, 0.0, s);
+    fill(st.ep, blk_elems(n, 6), 0.0, s);
     fill(st.g, n, d_.g0, s);
-    fill(st.gam, NS * n, 0.0, s);
+    fill(st.gam, blk_elems(n, NS), 0.0, s);
     fill(st.acc, n, 0.0, s);
     hipLaunchKernelGGL(k_elastic<K>, dim3(ceil_div(this->n_, kCBT)), dim3(kCBT), 0, s, this->n_,
                        this->idx_, this->npts_, rot9, C_, nullptr, nullptr, tangent, 0);
@@ -834,14 +833,16 @@ class EvpModel final : public Model<K> {
              hipStream_t s) override {
     const int n = this->n_;
     std::vector<double> soa;
+    // per-point arrays: one value per point, or nc components blocked by 64
+    // points (blk_idx; for nc = 1 the two layouts coincide)
     auto fetch = [&](const double* src, int nc) {
-      soa.resize(size_t(nc) * n);
+      soa.resize(nc == 1 ? size_t(n) : size_t(blk_elems(n, nc)));
       if (n) MTL_HIP(hipMemcpyAsync(soa.data(), src, soa.size() * sizeof(double),
                                     hipMemcpyDeviceToHost, s));
       stream_sync(s);
       out.resize(size_t(nc) * n);
       for (int i = 0; i < n; ++i)
-        for (int c = 0; c < nc; ++c) out[size_t(i) * nc + c] = soa[size_t(c) * n + i];
+        for (int c = 0; c < nc; ++c) out[size_t(i) * nc + c] = soa[size_t(blk_idx(c, i, nc))];
       ncomp = nc;
     };
     switch (which) {
