@@ -33,15 +33,20 @@ __device__ double sum_partials(const double* partials, int count, double* red) {
 }
 
 // r1 is zeroed although iteration 1 multiplies it by 0 (c_r1 = 0): stale
-// memory may hold inf/NaN bit patterns and 0 * inf would poison u.
+// memory may hold inf/NaN bit patterns and 0 * inf would poison u.  The
+// vectors hold 6 components blocked by 64 points (blk_idx); b is SoA [6][n].
+// Entries of the padded last block are set to zero and stay zero.
 template <typename T>
-__global__ void __launch_bounds__(kVBT) k_init(long long n6, const double* b, T* r1, T* r2, T* x,
-                                                T* w, T* w2, double* partials) {
+__global__ void __launch_bounds__(kVBT) k_init(long long nb6, long long n, const double* b, T* r1,
+                                                T* r2, T* x, T* w, T* w2, double* partials) {
   __shared__ double red[kVBT / 64];
   double acc = 0.0;
-  for (long long i = blockIdx.x * (long long)kVBT + threadIdx.x; i < n6;
+  for (long long i = blockIdx.x * (long long)kVBT + threadIdx.x; i < nb6;
        i += (long long)gridDim.x * kVBT) {
-    const T v = T(b[i]);
+    const long long q = i >> 6;
+    const long long p = (q / 6) * 64 + (i & 63);
+    const int c = int(q % 6);
+    const T v = p < n ? T(b[c * n + p]) : T(0);
     r1[i] = T(0);
     r2[i] = v;
     x[i] = T(0);
@@ -199,7 +204,7 @@ int vec_blocks(long long n6) {
 
 template <typename T, typename A>
 Minres<T, A>::Minres(Spectral<T, A>& sp) : sp_(sp) {
-  n6_ = 6 * sp.grid().n;
+  n6_ = blk_elems(sp.grid().n, 6);  // 6 components blocked by 64 points
   for (auto& v : vec_) MTL_HIP(hipMalloc(&v, n6_ * sizeof(T)));
   MTL_HIP(hipMalloc(&x_, n6_ * sizeof(T)));
   partial_cap_ = std::max(kVBlocks, sp.zinv_blocks());
@@ -228,7 +233,8 @@ int Minres<T, A>::solve(const double* b, const T* tangent, const Vec6& mask, dou
   T* w2 = vec_[3];
   T* wf = vec_[4];
   const int vb = vec_blocks(n6_);
-  hipLaunchKernelGGL(k_init<T>, dim3(vb), dim3(kVBT), 0, s, n6_, b, r1, r2, x_, w2, wf, partials_);
+  hipLaunchKernelGGL(k_init<T>, dim3(vb), dim3(kVBT), 0, s, n6_, sp_.grid().n, b, r1, r2, x_, w2,
+                     wf, partials_);
   MTL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_start, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb, rtol, maxiter);
   MTL_LAUNCH_CHECK();

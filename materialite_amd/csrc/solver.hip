@@ -78,14 +78,14 @@ __global__ void __launch_bounds__(kNBT) k_add_correction(long long n, const K* x
        p += (long long)gridDim.x * kNBT) {
 #pragma unroll
     for (int c = 0; c < 6; ++c) {
-      const double d = double(x[c * n + p]) + shift[c];
+      const double d = double(x[blk_idx(c, p, 6)]) + shift[c];  // x blocked by 64 points
       eps[c * n + p] += d;
       fluct[c * n + p] += d;
     }
   }
 }
 
-// per-block component sums of a [6][n] field
+// per-block component sums of a 6-component field blocked by 64 points
 template <typename K>
 __global__ void __launch_bounds__(kNBT) k_sum6(long long n, const K* x, double* partials) {
   __shared__ double red[kNBT / 64];
@@ -93,7 +93,7 @@ __global__ void __launch_bounds__(kNBT) k_sum6(long long n, const K* x, double* 
   for (long long p = blockIdx.x * (long long)kNBT + threadIdx.x; p < n;
        p += (long long)gridDim.x * kNBT)
 #pragma unroll
-    for (int c = 0; c < 6; ++c) acc[c] += double(x[c * n + p]);
+    for (int c = 0; c < 6; ++c) acc[c] += double(x[blk_idx(c, p, 6)]);
 #pragma unroll
   for (int c = 0; c < 6; ++c) {
     const double t = block_sum<kNBT>(acc[c], red);
@@ -130,7 +130,7 @@ __global__ void __launch_bounds__(kNBT) k_newton_norms(long long n, const double
       const double s = sig[c * n + p];
       ss[c] += s;
       ds[c] = s - sig_old[c * n + p];
-      xv[c] = x ? double(x[c * n + p]) : 0.0;
+      xv[c] = x ? double(x[blk_idx(c, p, 6)]) : 0.0;
     }
     const double tr = (ds[0] + ds[1] + ds[2]) / 3.0;
     double d2 = 0.0;
@@ -186,11 +186,14 @@ __global__ void k_scale(long long count, double* a, double f) {
     a[i] *= f;
 }
 
-template <typename From, typename To>
-__global__ void k_convert(long long count, const From* a, To* b) {
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < count;
-       i += (long long)gridDim.x * blockDim.x)
-    b[i] = To(a[i]);
+// SoA FP64 copy [6][n] of a 6-component field blocked by 64 points
+template <typename K>
+__global__ void k_unblock6(long long n, const K* a, double* b) {
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < 6 * n;
+       i += (long long)gridDim.x * blockDim.x) {
+    const long long c = i / n, p = i - c * n;
+    b[i] = double(a[blk_idx(int(c), p, 6)]);
+  }
 }
 
 // point-major copy of an SoA field: aos[p * nc + c] = soa[c * n + p]
@@ -590,8 +593,8 @@ class Solver final : public SolverBase {
     std::memcpy(mask.v, m, sizeof(mask.v));
     int istop = 0;
     const int itn = minres_->solve(b, cur_D(), mask, rtol, 5LL * 6LL * n_, s_, &istop);
-    hipLaunchKernelGGL((k_convert<K, double>), dim3(grid_for(6 * n_)), dim3(kNBT), 0, s_, 6 * n_,
-                       minres_->x(), x);
+    hipLaunchKernelGGL(k_unblock6<K>, dim3(grid_for(6 * n_)), dim3(kNBT), 0, s_, n_, minres_->x(),
+                       x);
     MTL_LAUNCH_CHECK();
     stream_sync(s_);
     return itn;

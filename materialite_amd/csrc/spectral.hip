@@ -121,7 +121,7 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan
       } else {
         T v[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) v[c] = s * T(in[c * g.n + p]);
+        for (int c = 0; c < 6; ++c) v[c] = s * T(in[blk_idx(c, p, 6)]);
 #pragma unroll
         for (int i2 = 0; i2 < 6; ++i2) {
           T acc = T(0);
@@ -540,7 +540,7 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan
         out[(2 * q) * g.n + p] = TOut(va);
         out[(2 * q + 1) * g.n + p] = TOut(vb);
       } else {
-        const long long ia = (2 * q) * g.n + p, ib = (2 * q + 1) * g.n + p;
+        const long long ia = blk_idx(2 * q, p, 6), ib = blk_idx(2 * q + 1, p, 6);
         const TS ua = TS(va - c1 * T(r1[ia]));
         const TS ub = TS(vb - c1 * T(r1[ib]));
         r1[ia] = ua;
