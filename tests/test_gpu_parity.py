@@ -173,6 +173,51 @@ def test_minres_matches_oracle(precision):
         assert np.abs(x - x_ref).max() <= 1e-3 * scale
 
 
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_minres_partial_point_block(precision):
+    """5 x 6 x 7 = 210 points: three 64-point blocks and a partial one.  The
+    Krylov vectors are stored blocked by 64 points; the padding of the last
+    block must stay out of every inner product and of the returned solution."""
+    from materialite_amd._native import NativeSolver
+    from oracle import constitutive as C
+    from oracle import tensors as OT
+    from oracle.minres import minres
+    from oracle.projection import apply_projection
+    from tests import fixtures as F
+
+    dims, n = (5, 6, 7), 210
+    print(f"minres partial block {precision}", flush=True)
+    rng = np.random.default_rng(210)
+    rot = OT.rotation_from_euler(OT.random_euler(n, rng))
+    tangent = C.Elastic(F.TI_STIFFNESS).initialize(rot)
+    mask = np.zeros(6)
+    sig = np.einsum("pmn,pn->pm", tangent, rng.normal(scale=1e-4, size=(n, 6)))
+    b = apply_projection(-sig, dims, stress_mask=mask)
+
+    def lhs(x):
+        return apply_projection(np.einsum("pmn,pn->pm", tangent, x.reshape(n, 6)), dims,
+                                stress_mask=mask).ravel()
+
+    rtol = 1e-7 if precision == "fp64" else 1e-5
+    x_ref, itn_ref, _ = minres(lhs, b.ravel(), rtol=rtol, maxiter=5 * 6 * n)
+    sol = NativeSolver(dims, precision)
+    sol.add_elastic(F.TI_STIFFNESS)
+    sol.set_microstructure(rot, np.zeros(n, dtype=np.int32))
+    sol.initialize()
+    bd = _dev(_soa(b))
+    xd = torch.empty_like(bd)
+    itn = sol.minres_dev(bd.data_ptr(), xd.data_ptr(), mask, rtol)
+    x = xd.cpu().numpy().T.ravel()
+    sol.close()
+    scale = np.abs(x_ref).max()
+    if precision == "fp64":
+        assert abs(itn - itn_ref) <= 1, (itn, itn_ref)
+        assert np.abs(x - x_ref).max() <= 1e-9 * scale
+    else:
+        assert itn <= 2 * itn_ref + 5, (itn, itn_ref)
+        assert np.abs(x - x_ref).max() <= 1e-3 * scale
+
+
 @pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-fft64"])
 @pytest.mark.parametrize("rate_exponent", [10.0, 10.5])
 def test_evp_update_matches_oracle(precision, rate_exponent):
