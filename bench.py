@@ -54,7 +54,7 @@ def parse():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--grid", type=int, default=128)
     ap.add_argument("--grains", type=int, default=1000)
-    ap.add_argument("--precision", default="mixed", choices=["mixed", "mixed-fft64", "fp64"])
+    ap.add_argument("--precision", default="refined", choices=["refined", "mixed", "fp64"])
     ap.add_argument("--compare-fp64", type=int, default=1,
                     help="also time the plain FP64 form on the same increments")
     ap.add_argument("--cpu-baseline", type=int, default=1)
@@ -252,12 +252,12 @@ def large_two_phase(n=256, grains=1000, steps=30, dt=0.2, converged=False):
     ls = LoadSchedule.from_constant_uniaxial_strain_rate(1e-3, "z")
     res = {}
     incs = {}
-    for prec in ("mixed", "mixed-fft64", "fp64"):
+    for prec in ("refined", "mixed", "fp64"):
         print(f"[bench] 256^3 two-phase: {prec}", file=sys.stderr, flush=True)
         res[prec] = _timed_run(m, ls, None, prec, steps * dt, dt, phase_label="phase")
         incs[prec] = max(_timed_run.increments, 1)
     (sm, km, tm), (sd, kd, td) = res["mixed"], res["fp64"]
-    sa, ka, ta = res["mixed-fft64"]
+    sa, ka, ta = res["refined"]
     vim, vid = n ** 3 * km / tm, n ** 3 * kd / td
     conv = None
     if converged:
@@ -286,7 +286,7 @@ def large_two_phase(n=256, grains=1000, steps=30, dt=0.2, converged=False):
             "max_rel_stress_error": float(np.abs(sa - sd).max() / np.abs(sd).max()),
             "stress_error_profile": stress_error_profile(sa, sd),
             "krylov": int(ka), "vi_per_s": n ** 3 * ka / ta,
-            "ms_per_increment": 1e3 * ta / incs["mixed-fft64"],
+            "ms_per_increment": 1e3 * ta / incs["refined"],
         },
         "converged_reference": conv,
     }
@@ -771,8 +771,7 @@ def main():
             max_rel_stress_error_of_mixed=float(
                 abs(main_rec["stress"] - tight["stress"]).max() / scale))
         if args.precision == "mixed":
-            # the same FP32 data flow with FP64 arithmetic inside the FFT passes
-            f64a = time_increments(material, "mixed-fft64", args.warmup, args.steps, profile=False)
+            f64a = time_increments(material, "refined", args.warmup, args.steps, profile=False)
             fp64["mixed_fft64_form"] = {
                 "value": npts * f64a["krylov"] / f64a["seconds"],
                 "ms_per_step": 1e3 * f64a["seconds"] / max(f64a["steps"], 1),

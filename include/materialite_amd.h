@@ -31,8 +31,11 @@ typedef struct mtl_solver mtl_solver;
 enum { MTL_FP64 = 0, /* every field, FFT and Krylov vector in FP64 */
        MTL_MIXED = 1, /* Krylov vectors, tangents and matvec FFTs in FP32;
                          strain/stress, residual FFT and all norms in FP64 */
-       MTL_MIXED_FFT64 = 2 /* as MTL_MIXED (FP32 data in device memory), with the
-                              matvec FFT butterflies, twiddles and Ghat in FP64 */ };
+       MTL_REFINED = 2 /* FP32 tangent and MINRES w / x vectors; the matvec
+                          spectrum and the Lanczos vectors stored as an FP32 value
+                          plus a bf16 remainder (32 significant bits in 6 bytes);
+                          FFT butterflies, twiddles and Ghat in FP64; strain/stress,
+                          residual FFT and all norms in FP64 */ };
 
 /* hardening laws (small_strain_fft/hardening_laws.py) */
 enum { MTL_HARDENING_PERFECT = 0, /* :18-25 */
@@ -59,6 +62,18 @@ const char* mtl_last_error(void);
  * _get_projection_operator (:301-368), which is never materialised here. */
 int mtl_solver_create(int nx, int ny, int nz, int precision, mtl_solver** out);
 int mtl_solver_destroy(mtl_solver* h);
+
+/* A handle over num_points material points and no grid, for the constitutive
+ * models' own protocol: initialize, calculate_stress_and_tangent and
+ * update_state_variables of Elastic (elastic.py:23-46), ElasticViscoplastic
+ * (elastic_viscoplastic.py:52-271) and IsotropicElasticPlastic
+ * (isotropic_elastic_plastic.py:55-179), several models at once as Multiphase
+ * (multiphase.py:31-62).  The model, microstructure, initialize, commit
+ * (update_state_variables; the scale is ignored), get_field and constitutive
+ * entry points apply; step, rollback, project and minres return an error.
+ * precision: MTL_FP64 keeps the tangent in FP64, the other forms in FP32.
+ * Destroy with mtl_solver_destroy. */
+int mtl_points_create(long long num_points, int precision, mtl_solver** out);
 
 /* Constitutive models (crystal-frame Mandel 6x6 stiffness, row-major).
  * Elastic: small_strain_fft/elastic.py:16-49. */
@@ -114,6 +129,8 @@ typedef struct {
   double equilibrium_error;
   double scaled_strain_error;
   double projected_scaled_strain_error;
+  double linear_solver_tolerance_used; /* the requested one, or the FP32 form's floor
+                                          (1e-5) when the request is tighter */
 } mtl_step_report;
 
 /* One load increment: predictor, Newton loop with MINRES corrections and
@@ -144,6 +161,14 @@ int mtl_solver_minres(mtl_solver* h, const double* b_dev, double* x_dev,
  * tangent to the handle.  iterations = 0 on non-convergence. */
 int mtl_solver_constitutive(mtl_solver* h, const double* strain_dev, const double* guess_dev,
                             double time_increment, double* stress_dev, int* iterations);
+
+/* calculate_stress_and_tangent on host arrays (a point handle): strain and
+ * guess stress N x 6 point-major, stress written N x 6 when the update
+ * converged; the tangent is then read with mtl_solver_get_field(h,
+ * MTL_FIELD_TANGENT).  iterations = 0 when a model did not converge (the
+ * reference returns (None, None, False) there, elastic_viscoplastic.py:166-168). */
+int mtl_solver_constitutive_host(mtl_solver* h, const double* strain, const double* guess,
+                                 double time_increment, double* stress, int* iterations);
 
 /* ---- kernel timing (no counterpart in the reference, which has no profiler
  * hooks; used by bench.py for the roofline figures) ----

@@ -16,8 +16,8 @@ LIB_PATH = os.environ.get("MATERIALITE_AMD_LIB", os.path.join(HERE, "libmaterial
 
 MTL_FP64 = 0
 MTL_MIXED = 1
-MTL_MIXED_FFT64 = 2
-PRECISIONS = {"fp64": MTL_FP64, "mixed": MTL_MIXED, "mixed-fft64": MTL_MIXED_FFT64}
+MTL_REFINED = 2
+PRECISIONS = {"fp64": MTL_FP64, "mixed": MTL_MIXED, "refined": MTL_REFINED}
 
 HARDENING = {
     "perfect_plasticity": 0,
@@ -68,6 +68,7 @@ class StepReport(ctypes.Structure):
         ("equilibrium_error", ctypes.c_double),
         ("scaled_strain_error", ctypes.c_double),
         ("projected_scaled_strain_error", ctypes.c_double),
+        ("linear_solver_tolerance_used", ctypes.c_double),
     ]
 
 
@@ -102,6 +103,11 @@ EXPORTS = {
     "mtl_solver_constitutive": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                                ctypes.c_double, ctypes.c_void_p,
                                                ctypes.POINTER(ctypes.c_int)]),
+    "mtl_points_create": (ctypes.c_int, [ctypes.c_longlong, ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_void_p)]),
+    "mtl_solver_constitutive_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
+                                                    ctypes.c_void_p, ctypes.c_double,
+                                                    ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
     "mtl_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "mtl_profile_reset": (ctypes.c_int, []),
     "mtl_profile_read": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_double),
@@ -287,3 +293,28 @@ class NativeSolver:
                                                 ctypes.c_void_p(guess_ptr), float(dt),
                                                 ctypes.c_void_p(stress_ptr), ctypes.byref(it)))
         return it.value
+
+
+class NativePoints(NativeSolver):
+    """Owning wrapper of a point handle (``mtl_points_create``): constitutive
+    models over ``num_points`` material points and no grid."""
+
+    def __init__(self, num_points, precision="fp64"):
+        self._lib = load_library()
+        if precision not in PRECISIONS:
+            raise ValueError(f"precision must be one of {sorted(PRECISIONS)}, not {precision!r}")
+        h = ctypes.c_void_p()
+        check(self._lib.mtl_points_create(int(num_points), PRECISIONS[precision], ctypes.byref(h)))
+        self._h = h
+        self.dims = (int(num_points), 1, 1)
+        self.num_points = int(num_points)
+
+    def constitutive_host(self, strain, guess, dt):
+        """(stress N x 6, iterations); iterations 0 when a model did not converge."""
+        e = np.ascontiguousarray(strain, dtype=np.float64).reshape(self.num_points, 6)
+        g = np.ascontiguousarray(guess, dtype=np.float64).reshape(self.num_points, 6)
+        out = np.empty((self.num_points, 6))
+        it = ctypes.c_int()
+        check(self._lib.mtl_solver_constitutive_host(self._h, _ptr(e), _ptr(g), float(dt), _ptr(out),
+                                                     ctypes.byref(it)))
+        return out, it.value

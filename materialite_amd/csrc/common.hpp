@@ -25,6 +25,9 @@ struct Error : std::runtime_error {
 
 #define MTL_LAUNCH_CHECK() MTL_HIP(hipGetLastError())
 
+// message returned by mtl_last_error() on this thread
+void set_last_error(const std::string& msg);
+
 // ------------------------------------------------------------------ complex
 template <typename T>
 struct cplx {

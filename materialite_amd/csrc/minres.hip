@@ -36,9 +36,11 @@ __device__ double sum_partials(const double* partials, int count, double* red) {
 // memory may hold inf/NaN bit patterns and 0 * inf would poison u.  The
 // vectors hold 6 components blocked by 64 points (blk_idx); b is SoA [6][n].
 // Entries of the padded last block are set to zero and stay zero.
-template <typename T>
-__global__ void __launch_bounds__(kVBT) k_init(long long nb6, long long n, const double* b, T* r1,
-                                                T* r2, T* x, T* w, T* w2, double* partials) {
+template <typename S>
+__global__ void __launch_bounds__(kVBT) k_init(long long nb6, long long n, const double* b,
+                                                RVec<S> r1, RVec<S> r2, work_t<S>* x,
+                                                work_t<S>* w, work_t<S>* w2, double* partials) {
+  using T = work_t<S>;
   __shared__ double red[kVBT / 64];
   double acc = 0.0;
   for (long long i = blockIdx.x * (long long)kVBT + threadIdx.x; i < nb6;
@@ -46,13 +48,14 @@ __global__ void __launch_bounds__(kVBT) k_init(long long nb6, long long n, const
     const long long q = i >> 6;
     const long long p = (q / 6) * 64 + (i & 63);
     const int c = int(q % 6);
-    const T v = p < n ? T(b[c * n + p]) : T(0);
-    r1[i] = T(0);
-    r2[i] = v;
+    const double v = p < n ? b[c * n + p] : 0.0;
+    r1.put(i, 0.0);
+    r2.put(i, v);
     x[i] = T(0);
     w[i] = T(0);
     w2[i] = T(0);
-    acc += double(v) * double(v);
+    const double vs = r2.template get<double>(i);  // as stored
+    acc += vs * vs;
   }
   const double t = block_sum<kVBT>(acc, red);
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
@@ -93,17 +96,18 @@ __global__ void __launch_bounds__(kSBT) k_alpha(MinresState* st, const double* p
   st->c_r2 = a / st->beta;
 }
 
-template <typename T>
-__global__ void __launch_bounds__(kVBT) k_lanczos(long long n6, const MinresState* st, T* u,
-                                                   const T* r2, double* partials) {
+// T: arithmetic (FP32 in the FP32 form, FP64 otherwise)
+template <typename S, typename T>
+__global__ void __launch_bounds__(kVBT) k_lanczos(long long n6, const MinresState* st, RVec<S> u,
+                                                   RVec<S> r2, double* partials) {
   __shared__ double red[kVBT / 64];
   if (st->done) return;
   const T c = T(st->c_r2);
   double acc = 0.0;
   for (long long i = blockIdx.x * (long long)kVBT + threadIdx.x; i < n6;
        i += (long long)gridDim.x * kVBT) {
-    const T y = u[i] - c * r2[i];
-    u[i] = y;
+    const T y = u.template get<T>(i) - c * r2.template get<T>(i);
+    u.put(i, y);
     acc += double(y) * double(y);
   }
   const double t = block_sum<kVBT>(acc, red);
@@ -147,10 +151,12 @@ __global__ void __launch_bounds__(kSBT) k_givens(MinresState* st, const double* 
   *st = z;
 }
 
-template <typename T>
+template <typename S, typename T>
 __global__ void __launch_bounds__(kVBT) k_update(long long n6, const MinresState* st,
-                                                  const T* r2old, const T* w1, const T* w2,
-                                                  T* wnew, T* x, double* partials) {
+                                                  RVec<S> r2old, const work_t<S>* w1,
+                                                  const work_t<S>* w2, work_t<S>* wnew,
+                                                  work_t<S>* x, double* partials) {
+  using W = work_t<S>;
   __shared__ double red[kVBT / 64];
   if (st->done) return;
   const T sv = T(st->s_prev), oe = T(st->oldeps), de = T(st->delta), dn = T(st->denom),
@@ -158,10 +164,10 @@ __global__ void __launch_bounds__(kVBT) k_update(long long n6, const MinresState
   double acc = 0.0;
   for (long long i = blockIdx.x * (long long)kVBT + threadIdx.x; i < n6;
        i += (long long)gridDim.x * kVBT) {
-    const T v = sv * r2old[i];
-    const T w = (v - oe * w1[i] - de * w2[i]) * dn;
+    const T v = sv * r2old.template get<T>(i);
+    const W w = W((v - oe * T(w1[i]) - de * T(w2[i])) * dn);
     wnew[i] = w;
-    const T xn = x[i] + ph * w;
+    const W xn = W(T(x[i]) + ph * T(w));
     x[i] = xn;
     acc += double(xn) * double(xn);
   }
@@ -202,20 +208,26 @@ int vec_blocks(long long n6) {
 
 }  // namespace
 
-template <typename T, typename A>
-Minres<T, A>::Minres(Spectral<T, A>& sp) : sp_(sp) {
+template <typename S, typename A>
+Minres<S, A>::Minres(Spectral<S, A>& sp) : sp_(sp) {
   n6_ = blk_elems(sp.grid().n, 6);  // 6 components blocked by 64 points
-  for (auto& v : vec_) MTL_HIP(hipMalloc(&v, n6_ * sizeof(T)));
-  MTL_HIP(hipMalloc(&x_, n6_ * sizeof(T)));
+  for (int k = 0; k < 2; ++k) {
+    MTL_HIP(hipMalloc(&lanczos_mem_[k], real_bytes<S>(n6_)));
+    lanczos_[k] = rvec_over<S>(lanczos_mem_[k], n6_);
+  }
+  for (auto& v : w_) MTL_HIP(hipMalloc(&v, n6_ * sizeof(W)));
+  MTL_HIP(hipMalloc(&x_, n6_ * sizeof(W)));
   partial_cap_ = std::max(kVBlocks, sp.zinv_blocks());
   MTL_HIP(hipMalloc(&partials_, partial_cap_ * sizeof(double)));
   MTL_HIP(hipMalloc(&state_, sizeof(MinresState)));
   MTL_HIP(hipHostMalloc(&host_state_, sizeof(MinresState), hipHostMallocDefault));
 }
 
-template <typename T, typename A>
-Minres<T, A>::~Minres() {
-  for (auto v : vec_)
+template <typename S, typename A>
+Minres<S, A>::~Minres() {
+  for (void* v : lanczos_mem_)
+    if (v) (void)hipFree(v);
+  for (auto v : w_)
     if (v) (void)hipFree(v);
   if (x_) (void)hipFree(x_);
   if (partials_) (void)hipFree(partials_);
@@ -223,17 +235,17 @@ Minres<T, A>::~Minres() {
   if (host_state_) (void)hipHostFree(host_state_);
 }
 
-template <typename T, typename A>
-int Minres<T, A>::solve(const double* b, const T* tangent, const Vec6& mask, double rtol,
+template <typename S, typename A>
+int Minres<S, A>::solve(const double* b, const W* tangent, const Vec6& mask, double rtol,
                      long long maxiter, hipStream_t s, int* istop_out) {
   // roles: r1, r2 (Lanczos), w1, w2 (previous w), wf (free slot for the new w)
-  T* r1 = vec_[0];
-  T* r2 = vec_[1];
-  T* w1 = vec_[2];
-  T* w2 = vec_[3];
-  T* wf = vec_[4];
+  RVec<S> r1 = lanczos_[0];
+  RVec<S> r2 = lanczos_[1];
+  W* w1 = w_[0];
+  W* w2 = w_[1];
+  W* wf = w_[2];
   const int vb = vec_blocks(n6_);
-  hipLaunchKernelGGL(k_init<T>, dim3(vb), dim3(kVBT), 0, s, n6_, sp_.grid().n, b, r1, r2, x_, w2,
+  hipLaunchKernelGGL(k_init<S>, dim3(vb), dim3(kVBT), 0, s, n6_, sp_.grid().n, b, r1, r2, x_, w2,
                      wf, partials_);
   MTL_LAUNCH_CHECK();
   hipLaunchKernelGGL(k_start, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb, rtol, maxiter);
@@ -250,19 +262,19 @@ int Minres<T, A>::solve(const double* b, const T* tangent, const Vec6& mask, dou
       const int np = sp_.matvec_minres(r2, tangent, r1, mask, state_, partials_, s);
       profiler().launch(kKMinresScalar, k_alpha, dim3(1), dim3(kSBT), 0, s, state_, partials_, np);
       MTL_LAUNCH_CHECK();
-      profiler().launch(kKMinresVector, k_lanczos<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, r2, partials_);
+      profiler().launch(kKMinresVector, k_lanczos<S, A>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, r2, partials_);
       MTL_LAUNCH_CHECK();
       profiler().launch(kKMinresScalar, k_givens, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb);
       MTL_LAUNCH_CHECK();
       // Lanczos roles rotate: r1 <- r2 (old), r2 <- y (stored in old r1)
       std::swap(r1, r2);
       // v_k = r2_old / beta_k now lives in r1
-      profiler().launch(kKMinresVector, k_update<T>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, w1, w2, wf, x_, partials_);
+      profiler().launch(kKMinresVector, k_update<S, A>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, w1, w2, wf, x_, partials_);
       MTL_LAUNCH_CHECK();
       profiler().launch(kKMinresScalar, k_tests, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb);
       MTL_LAUNCH_CHECK();
       // w roles rotate: w1 <- w2, w2 <- new w, free slot <- old w1
-      T* old_w1 = w1;
+      W* old_w1 = w1;
       w1 = w2;
       w2 = wf;
       wf = old_w1;
@@ -294,6 +306,6 @@ int Minres<T, A>::solve(const double* b, const T* tangent, const Vec6& mask, dou
 
 template class Minres<float>;
 template class Minres<double>;
-template class Minres<float, double>;
+template class Minres<f48, double>;
 
 }  // namespace mtl

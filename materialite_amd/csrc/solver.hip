@@ -218,10 +218,11 @@ __global__ void k_aos_to_soa(long long n, int nc, const double* aos, double* soa
   }
 }
 
-// lowest relative MINRES tolerance of the mixed forms: FP32 vectors with FP32
-// FFT arithmetic, and with FP64 FFT arithmetic
+// lowest relative MINRES tolerance of the FP32 form: its FP32 Krylov vectors
+// and FFT arithmetic cannot honour tighter ones on the singular pore systems
+// (the refined form, whose Lanczos vectors and spectrum keep 32 significant
+// bits, has no floor)
 constexpr double kMixedKrylovFloor = 1.0e-5;
-constexpr double kMixedFft64KrylovFloor = 1.0e-6;
 
 bool debug_enabled() {
   static const bool on = [] {
@@ -239,6 +240,119 @@ thread_local std::string g_last_error;
 
 }  // namespace
 
+// The constitutive models of a handle over its n points: per-point rotations
+// (FP64 SoA [9][n]) and one Model per registered description, each over the
+// points whose model id names it (Multiphase dispatch, multiphase.py:31-95).
+template <typename K>
+class ModelSet {
+ public:
+  explicit ModelSet(long long n) : n_(n) { MTL_HIP(hipMalloc(&rot_, 9 * n * sizeof(double))); }
+  ~ModelSet() {
+    models_.clear();
+    if (rot_) (void)hipFree(rot_);
+  }
+  ModelSet(const ModelSet&) = delete;
+  ModelSet& operator=(const ModelSet&) = delete;
+
+  int add(const ModelDesc& d) {
+    if (d.kind == 1 && (d.nslip < 1 || int(d.schmid.size()) != 6 * d.nslip))
+      throw Error("bad slip-system table");
+    if (d.kind == 1 && !evp_slip_count_built(d.nslip))
+      throw Error("elasto-viscoplastic model with " + std::to_string(d.nslip) +
+                  " slip systems is not built (supported: 1, 2, 3, 4, 6, 9, 12, 15, 18, 24)");
+    descs_.push_back(d);
+    return int(descs_.size()) - 1;
+  }
+
+  void set_microstructure(const double* rot, const int32_t* mop, hipStream_t s) {
+    // point-major rotations are uploaded as they are and reordered to SoA on
+    // the device (a stream-ordered temporary: no device-wide synchronisation)
+    double* aos = nullptr;
+    const size_t bytes = 9 * size_t(n_) * sizeof(double);
+    MTL_HIP(hipMallocAsync(reinterpret_cast<void**>(&aos), bytes, s));
+    MTL_HIP(hipMemcpyAsync(aos, rot, bytes, hipMemcpyHostToDevice, s));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(9 * n_)), dim3(kNBT), 0, s, n_, 9,
+                       static_cast<const double*>(aos), rot_);
+    MTL_LAUNCH_CHECK();
+    MTL_HIP(hipFreeAsync(aos, s));
+    stream_sync(s);
+    std::vector<std::vector<int>> pts(descs_.size());
+    for (long long p = 0; p < n_; ++p) {
+      const int m = mop ? mop[p] : 0;
+      if (m < 0 || m >= int(descs_.size()))
+        throw Error("model id out of range at point " + std::to_string(p));
+      pts[m].push_back(int(p));
+    }
+    models_.clear();
+    for (size_t m = 0; m < descs_.size(); ++m)
+      models_.push_back(make_model<K>(descs_[m], pts[m], n_));
+  }
+
+  // initial tangent of every point and zeroed model state
+  void init_tangent(K* D, hipStream_t s) {
+    if (models_.empty()) throw Error("set the microstructure before initialize");
+    MTL_HIP(hipMemsetAsync(D, 0, tan_elems(n_) * sizeof(K), s));
+    for (auto& m : models_) m->init_tangent(rot_, D, s);
+  }
+
+  // every model on its points (multiphase.py:38-51): 0 as soon as one fails,
+  // else the largest iteration count
+  int calculate(const double* eps, const double* guess, double dt, double* sig, K* D,
+                hipStream_t s) {
+    int iters = 0;
+    for (auto& m : models_) {
+      const int it = m->calculate(rot_, eps, guess, dt, sig, D, s);
+      if (!it) return 0;
+      iters = std::max(iters, it);
+    }
+    return iters;
+  }
+
+  void commit(hipStream_t s) {
+    for (auto& m : models_) m->commit(rot_, s);
+  }
+
+  // a state field scattered to the points of the models that have it, zero
+  // elsewhere (multiphase.py:75-95)
+  void state_field(int which, double* out, int* ncomp, hipStream_t s) {
+    int width = 0;
+    std::vector<std::pair<const Model<K>*, std::vector<double>>> parts;
+    for (auto& m : models_) {
+      std::vector<double> v;
+      int nc = 0;
+      if (m->field(which, rot_, v, nc, s)) {
+        width = std::max(width, nc);
+        parts.emplace_back(m.get(), std::move(v));
+      }
+    }
+    if (width == 0) throw Error("field not available in any constitutive model");
+    std::fill(out, out + size_t(width) * n_, 0.0);
+    for (auto& pr : parts) {
+      const auto& pts = pr.first->points();
+      const size_t nc = pr.second.size() / std::max<size_t>(pts.size(), 1);
+      for (size_t i = 0; i < pts.size(); ++i)
+        for (size_t c = 0; c < nc; ++c) out[size_t(pts[i]) * width + c] = pr.second[i * nc + c];
+    }
+    *ncomp = width;
+  }
+
+ private:
+  long long n_;
+  double* rot_ = nullptr;
+  std::vector<ModelDesc> descs_;
+  std::vector<std::unique_ptr<Model<K>>> models_;
+};
+
+// full Mandel 6x6 per point (N x 36) of a tangent [n/64][21][64]
+template <typename K>
+void tangent_to_host(const K* D, long long n, double* out) {
+  std::vector<K> blk(tan_elems(n));
+  MTL_HIP(hipMemcpy(blk.data(), D, blk.size() * sizeof(K), hipMemcpyDeviceToHost));
+  for (long long p = 0; p < n; ++p)
+    for (int a = 0; a < 6; ++a)
+      for (int b = 0; b < 6; ++b) out[p * 36 + a * 6 + b] = double(blk[tan_idx(sym6(a, b), p)]);
+}
+
 struct SolverBase {
   virtual ~SolverBase() = default;
   virtual int add_model(const ModelDesc& d) = 0;
@@ -251,25 +365,30 @@ struct SolverBase {
   virtual void project(const double* in, double* out, const double mask[6]) = 0;
   virtual int minres(const double* b, double* x, const double mask[6], double rtol) = 0;
   virtual int constitutive(const double* eps, const double* guess, double dt, double* sig) = 0;
+  // constitutive() on point-major host arrays (N x 6); point handles only
+  virtual int constitutive_host(const double*, const double*, double, double*) {
+    throw Error("host-array constitutive calls need a point handle (mtl_points_create)");
+  }
 };
 
-// K: tangent and Krylov vectors; A: arithmetic of the matvec FFT passes.
-template <typename K, typename A = K>
+// S: storage of the matvec spectrum and Lanczos vectors; K = work_t<S>: the
+// tangent and the MINRES w / x vectors; A: arithmetic of the matvec FFT passes.
+template <typename S, typename A = S>
 class Solver final : public SolverBase {
  public:
-  Solver(int nx, int ny, int nz) : sp64_(nx, ny, nz) {
-    if constexpr (std::is_same<Spectral<K, A>, Spectral<double>>::value) {
+  using K = work_t<S>;
+  Solver(int nx, int ny, int nz) : sp64_(nx, ny, nz), ms_(sp64_.grid().n) {
+    if constexpr (std::is_same<S, double>::value) {
       spk_ = &sp64_;  // one spectrum buffer for the residual and the matvec
     } else {
-      own_spk_ = std::make_unique<Spectral<K, A>>(nx, ny, nz);
+      own_spk_ = std::make_unique<Spectral<S, A>>(nx, ny, nz);
       spk_ = own_spk_.get();
     }
-    minres_ = std::make_unique<Minres<K, A>>(*spk_);
+    minres_ = std::make_unique<Minres<S, A>>(*spk_);
     n_ = sp64_.grid().n;
     MTL_HIP(hipStreamCreate(&s_));
     auto dalloc = [&](double*& p, long long c) { MTL_HIP(hipMalloc(&p, c * sizeof(double))); };
     for (double** f : {&eps_, &sig_, &sig_old_, &eps_old_, &fluct_, &guess_, &b_}) dalloc(*f, 6 * n_);
-    dalloc(rot_, 9 * n_);
     MTL_HIP(hipMalloc(&D_, tan_elems(n_) * sizeof(K)));
     MTL_HIP(hipMalloc(&D_old_, tan_elems(n_) * sizeof(K)));
     dalloc(partials_, kNormBlocks * 21);
@@ -277,8 +396,7 @@ class Solver final : public SolverBase {
     MTL_HIP(hipHostMalloc(&hpart_, kNormBlocks * 21 * sizeof(double), hipHostMallocDefault));
   }
   ~Solver() override {
-    models_.clear();
-    for (double* p : {eps_, sig_, sig_old_, eps_old_, fluct_, guess_, b_, rot_, partials_, shift_})
+    for (double* p : {eps_, sig_, sig_old_, eps_old_, fluct_, guess_, b_, partials_, shift_})
       if (p) (void)hipFree(p);
     if (D_) (void)hipFree(D_);
     if (D_old_) (void)hipFree(D_old_);
@@ -286,44 +404,16 @@ class Solver final : public SolverBase {
     if (s_) (void)hipStreamDestroy(s_);
   }
 
-  int add_model(const ModelDesc& d) override {
-    if (d.kind == 1 && (d.nslip < 1 || int(d.schmid.size()) != 6 * d.nslip))
-      throw Error("bad slip-system table");
-    if (d.kind == 1 && !evp_slip_count_built(d.nslip))
-      throw Error("elasto-viscoplastic model with " + std::to_string(d.nslip) +
-                  " slip systems is not built (supported: 1, 2, 3, 4, 6, 9, 12, 15, 18, 24)");
-    descs_.push_back(d);
-    return int(descs_.size()) - 1;
-  }
+  int add_model(const ModelDesc& d) override { return ms_.add(d); }
 
   void set_microstructure(const double* rot, const int32_t* mop) override {
-    // point-major rotations are uploaded as they are and reordered to SoA on
-    // the device (a stream-ordered temporary: no device-wide synchronisation)
-    double* aos = nullptr;
-    const size_t bytes = 9 * size_t(n_) * sizeof(double);
-    MTL_HIP(hipMallocAsync(reinterpret_cast<void**>(&aos), bytes, s_));
-    MTL_HIP(hipMemcpyAsync(aos, rot, bytes, hipMemcpyHostToDevice, s_));
-    hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(9 * n_)), dim3(kNBT), 0, s_, n_, 9,
-                       static_cast<const double*>(aos), rot_);
-    MTL_LAUNCH_CHECK();
-    MTL_HIP(hipFreeAsync(aos, s_));
-    stream_sync(s_);
-    std::vector<std::vector<int>> pts(descs_.size());
-    for (long long p = 0; p < n_; ++p) {
-      const int m = mop ? mop[p] : 0;
-      if (m < 0 || m >= int(descs_.size())) throw Error("model id out of range at point " + std::to_string(p));
-      pts[m].push_back(int(p));
-    }
-    models_.clear();
-    for (size_t m = 0; m < descs_.size(); ++m) models_.push_back(make_model<K>(descs_[m], pts[m], n_));
+    ms_.set_microstructure(rot, mop, s_);
   }
 
   void initialize() override {
-    if (models_.empty()) throw Error("set the microstructure before initialize");
+    ms_.init_tangent(D_, s_);
     for (double* f : {eps_, sig_, sig_old_, eps_old_, fluct_, guess_, b_})
       MTL_HIP(hipMemsetAsync(f, 0, 6 * n_ * sizeof(double), s_));
-    MTL_HIP(hipMemsetAsync(D_, 0, tan_elems(n_) * sizeof(K), s_));
-    for (auto& m : models_) m->init_tangent(rot_, D_, s_);
     MTL_HIP(hipMemcpyAsync(D_old_, D_, tan_elems(n_) * sizeof(K), hipMemcpyDeviceToDevice, s_));
     sig_stale_ = eps_stale_ = d_stale_ = guess_is_stress_ = false;
     stream_sync(s_);
@@ -331,14 +421,8 @@ class Solver final : public SolverBase {
 
   // Multiphase dispatch (multiphase.py:38-51): fails as soon as a model fails
   int constitutive_all(const double* eps, const double* guess, double dt, double* sig) {
-    int iters = 0;
     d_stale_ = false;  // D_ is rewritten (or restored on failure)
-    for (auto& m : models_) {
-      const int it = m->calculate(rot_, eps, guess, dt, sig, D_, s_);
-      if (!it) return 0;
-      iters = std::max(iters, it);
-    }
-    return iters;
+    return ms_.calculate(eps, guess, dt, sig, D_, s_);
   }
 
   struct Norms {
@@ -434,10 +518,9 @@ class Solver final : public SolverBase {
     // Newton iteration is an FP64-residual refinement step and the converged
     // state is the FP64 one up to the requested global tolerance.
     constexpr bool kMixed = !std::is_same<K, double>::value;
-    constexpr double kFloor =
-        std::is_same<A, double>::value ? kMixedFft64KrylovFloor : kMixedKrylovFloor;
-    const double lin_tol =
-        kMixed ? std::max(P.linear_solver_tolerance, kFloor) : P.linear_solver_tolerance;
+    constexpr double kFloor = std::is_same<S, float>::value ? kMixedKrylovFloor : 0.0;
+    const double lin_tol = std::max(P.linear_solver_tolerance, kFloor);
+    R.linear_solver_tolerance_used = lin_tol;
     bool converged = false;
     int iteration = 0;
     int max_constit = 0;
@@ -526,7 +609,7 @@ class Solver final : public SolverBase {
   }
 
   void commit(double scale) override {
-    for (auto& m : models_) m->commit(rot_, s_);
+    ms_.commit(s_);
     // the current stress, strain and tangent become the committed ones by
     // swapping buffers; until the next step rewrites them the current values
     // are read from the committed buffers (cur_*)
@@ -550,35 +633,12 @@ class Solver final : public SolverBase {
       return;
     }
     if (which == kFieldTangent) {
-      std::vector<K> blk(tan_elems(n_));
-      MTL_HIP(hipMemcpy(blk.data(), cur_D(), blk.size() * sizeof(K), hipMemcpyDeviceToHost));
-      for (long long p = 0; p < n_; ++p)
-        for (int a = 0; a < 6; ++a)
-          for (int b = 0; b < 6; ++b) out[p * 36 + a * 6 + b] = double(blk[tan_idx(sym6(a, b), p)]);
+      stream_sync(s_);
+      tangent_to_host(cur_D(), n_, out);
       *ncomp = 36;
       return;
     }
-    // state fields: scatter each model's rows; points of models without the
-    // field are zero (multiphase.py:75-95)
-    int width = 0;
-    std::vector<std::pair<const Model<K>*, std::vector<double>>> parts;
-    for (auto& m : models_) {
-      std::vector<double> v;
-      int nc = 0;
-      if (m->field(which, rot_, v, nc, s_)) {
-        width = std::max(width, nc);
-        parts.emplace_back(m.get(), std::move(v));
-      }
-    }
-    if (width == 0) throw Error("field not available in any constitutive model");
-    std::fill(out, out + size_t(width) * n_, 0.0);
-    for (auto& pr : parts) {
-      const auto& pts = pr.first->points();
-      const size_t nc = pr.second.size() / std::max<size_t>(pts.size(), 1);
-      for (size_t i = 0; i < pts.size(); ++i)
-        for (size_t c = 0; c < nc; ++c) out[size_t(pts[i]) * width + c] = pr.second[i * nc + c];
-    }
-    *ncomp = width;
+    ms_.state_field(which, out, ncomp, s_);
   }
 
   void project(const double* in, double* out, const double m[6]) override {
@@ -625,13 +685,13 @@ class Solver final : public SolverBase {
   }
 
   Spectral<double> sp64_;
-  std::unique_ptr<Spectral<K, A>> own_spk_;
-  Spectral<K, A>* spk_ = nullptr;
-  std::unique_ptr<Minres<K, A>> minres_;
+  std::unique_ptr<Spectral<S, A>> own_spk_;
+  Spectral<S, A>* spk_ = nullptr;
+  std::unique_ptr<Minres<S, A>> minres_;
   long long n_ = 0;
   hipStream_t s_ = nullptr;
   double *eps_ = nullptr, *sig_ = nullptr, *sig_old_ = nullptr, *eps_old_ = nullptr,
-         *fluct_ = nullptr, *guess_ = nullptr, *b_ = nullptr, *rot_ = nullptr;
+         *fluct_ = nullptr, *guess_ = nullptr, *b_ = nullptr;
   K *D_ = nullptr, *D_old_ = nullptr;
   // guess_ holds the stress of the last accepted constitutive call (the
   // buffers were swapped instead of copied) rather than the predictor's guess
@@ -645,14 +705,110 @@ class Solver final : public SolverBase {
   double* partials_ = nullptr;
   double* shift_ = nullptr;  // mean shift of the mixed-form correction (6)
   double* hpart_ = nullptr;
-  std::vector<ModelDesc> descs_;
-  std::vector<std::unique_ptr<Model<K>>> models_;
+  ModelSet<K> ms_;
+};
+
+// Constitutive models over n material points and no grid: the handle behind
+// the models' own protocol (initialize, calculate_stress_and_tangent,
+// update_state_variables of elastic.py:23-46, elastic_viscoplastic.py:52-271,
+// isotropic_elastic_plastic.py:55-179).  K: tangent storage.
+template <typename K>
+class Points final : public SolverBase {
+ public:
+  explicit Points(long long n) : n_(n), ms_(n) {
+    if (n < 1) throw Error("a point handle needs at least one point");
+    MTL_HIP(hipStreamCreate(&s_));
+    for (double** f : {&eps_, &guess_, &sig_, &aos_})
+      MTL_HIP(hipMalloc(f, 6 * size_t(n) * sizeof(double)));
+    MTL_HIP(hipMalloc(&D_, tan_elems(n) * sizeof(K)));
+  }
+  ~Points() override {
+    for (double* p : {eps_, guess_, sig_, aos_})
+      if (p) (void)hipFree(p);
+    if (D_) (void)hipFree(D_);
+    if (s_) (void)hipStreamDestroy(s_);
+  }
+
+  int add_model(const ModelDesc& d) override { return ms_.add(d); }
+  void set_microstructure(const double* rot, const int32_t* mop) override {
+    ms_.set_microstructure(rot, mop, s_);
+  }
+  void initialize() override {
+    ms_.init_tangent(D_, s_);
+    MTL_HIP(hipMemsetAsync(sig_, 0, 6 * size_t(n_) * sizeof(double), s_));
+    stream_sync(s_);
+  }
+  void step(const mtl_step_params&, mtl_step_report&) override { no_grid(); }
+  void commit(double) override {
+    ms_.commit(s_);
+    stream_sync(s_);
+  }
+  void rollback() override { no_grid(); }
+  void get_field(int which, double* out, int* ncomp) override {
+    if (which == kFieldStress) {
+      download6(sig_, out);
+      *ncomp = 6;
+    } else if (which == kFieldStrain) {
+      download6(eps_, out);
+      *ncomp = 6;
+    } else if (which == kFieldTangent) {
+      stream_sync(s_);
+      tangent_to_host(D_, n_, out);
+      *ncomp = 36;
+    } else {
+      ms_.state_field(which, out, ncomp, s_);
+    }
+  }
+  void project(const double*, double*, const double*) override { no_grid(); }
+  int minres(const double*, double*, const double*, double) override {
+    no_grid();
+    return 0;
+  }
+  int constitutive(const double* eps, const double* guess, double dt, double* sig) override {
+    const int it = ms_.calculate(eps, guess, dt, sig, D_, s_);
+    stream_sync(s_);
+    return it;
+  }
+  int constitutive_host(const double* eps, const double* guess, double dt, double* sig) override {
+    upload6(eps, eps_);
+    upload6(guess, guess_);
+    const int it = ms_.calculate(eps_, guess_, dt, sig_, D_, s_);
+    stream_sync(s_);
+    if (it) download6(sig_, sig);
+    return it;
+  }
+
+ private:
+  [[noreturn]] static void no_grid() {
+    throw Error("a point handle has no grid: step, rollback, project and minres need "
+                "mtl_solver_create");
+  }
+  void upload6(const double* host, double* soa) {
+    const size_t bytes = 6 * size_t(n_) * sizeof(double);
+    MTL_HIP(hipMemcpyAsync(aos_, host, bytes, hipMemcpyHostToDevice, s_));
+    hipLaunchKernelGGL(k_aos_to_soa, dim3(grid_for(6 * n_)), dim3(kNBT), 0, s_, n_, 6,
+                       static_cast<const double*>(aos_), soa);
+    MTL_LAUNCH_CHECK();
+  }
+  void download6(const double* soa, double* host) {
+    hipLaunchKernelGGL(k_soa_to_aos, dim3(grid_for(6 * n_)), dim3(kNBT), 0, s_, n_, 6, soa, aos_);
+    MTL_LAUNCH_CHECK();
+    MTL_HIP(hipMemcpyAsync(host, aos_, 6 * size_t(n_) * sizeof(double), hipMemcpyDeviceToHost, s_));
+    stream_sync(s_);
+  }
+  long long n_;
+  hipStream_t s_ = nullptr;
+  double *eps_ = nullptr, *guess_ = nullptr, *sig_ = nullptr, *aos_ = nullptr;
+  K* D_ = nullptr;
+  ModelSet<K> ms_;
 };
 
 Profiler& profiler() {
   static Profiler p;
   return p;
 }
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
 
 bool inverse6_host(const double* A, double* out) {
   double m[6][12];
@@ -712,7 +868,7 @@ const char* mtl_last_error(void) { return mtl::g_last_error.c_str(); }
 
 int mtl_solver_create(int nx, int ny, int nz, int precision, mtl_solver** out) {
   if (!out) return bad("null output handle");
-  if (precision != MTL_FP64 && precision != MTL_MIXED && precision != MTL_MIXED_FFT64)
+  if (precision != MTL_FP64 && precision != MTL_MIXED && precision != MTL_REFINED)
     return bad("unknown precision");
   return guarded([&] {
     auto h = std::make_unique<mtl_solver>();
@@ -721,7 +877,22 @@ int mtl_solver_create(int nx, int ny, int nz, int precision, mtl_solver** out) {
     else if (precision == MTL_MIXED)
       h->impl = std::make_unique<mtl::Solver<float>>(nx, ny, nz);
     else
-      h->impl = std::make_unique<mtl::Solver<float, double>>(nx, ny, nz);
+      h->impl = std::make_unique<mtl::Solver<mtl::f48, double>>(nx, ny, nz);
+    *out = h.release();
+  });
+}
+
+int mtl_points_create(long long num_points, int precision, mtl_solver** out) {
+  if (!out) return bad("null output handle");
+  if (precision != MTL_FP64 && precision != MTL_MIXED && precision != MTL_REFINED)
+    return bad("unknown precision");
+  if (num_points < 1 || num_points > (1LL << 31) - 1) return bad("number of points out of range");
+  return guarded([&] {
+    auto h = std::make_unique<mtl_solver>();
+    if (precision == MTL_FP64)
+      h->impl = std::make_unique<mtl::Points<double>>(num_points);
+    else
+      h->impl = std::make_unique<mtl::Points<float>>(num_points);
     *out = h.release();
   });
 }
@@ -842,6 +1013,13 @@ int mtl_profile_durations(int kernel, float* ms_out, long long capacity, long lo
   if (!count || (capacity > 0 && !ms_out)) return bad("null argument");
   if (kernel < 0 || kernel >= mtl::kNumKernelIds) return bad("unknown kernel id");
   return guarded([&] { *count = mtl::profiler().read_durations(kernel, ms_out, capacity); });
+}
+
+int mtl_solver_constitutive_host(mtl_solver* h, const double* strain, const double* guess,
+                                 double time_increment, double* stress, int* iterations) {
+  if (!h || !strain || !guess || !stress || !iterations) return bad("null argument");
+  return guarded(
+      [&] { *iterations = h->impl->constitutive_host(strain, guess, time_increment, stress); });
 }
 
 int mtl_solver_constitutive(mtl_solver* h, const double* strain_dev, const double* guess_dev,

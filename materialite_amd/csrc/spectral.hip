@@ -69,40 +69,20 @@ template <typename T, typename TS>
 __device__ __forceinline__ cplx<T> cvt(const cplx<TS>& v) {
   return {T(v.x), T(v.y)};
 }
-// two adjacent complex values, global FP32 storage <-> LDS (16 B accesses)
-template <typename T>
-__device__ __forceinline__ void load_pair(cplx<T>* dst, const cplx<float>* src) {
-  const float4 v = *reinterpret_cast<const float4*>(src);
-  if constexpr (sizeof(T) == 4) {
-    *reinterpret_cast<float4*>(dst) = v;
-  } else {
-    dst[0] = {T(v.x), T(v.y)};
-    dst[1] = {T(v.z), T(v.w)};
-  }
-}
-template <typename T>
-__device__ __forceinline__ void store_pair(cplx<float>* dst, const cplx<T>* src) {
-  if constexpr (sizeof(T) == 4) {
-    *reinterpret_cast<float4*>(dst) = *reinterpret_cast<const float4*>(src);
-  } else {
-    *reinterpret_cast<float4*>(dst) =
-        make_float4(float(src[0].x), float(src[0].y), float(src[1].x), float(src[1].y));
-  }
-}
-
 // ----------------------------------------------------------------- z forward
-template <typename T, typename TS, int MODE, typename TIn, bool BIG>
+template <typename T, typename S, int MODE, typename In, bool BIG>
 __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan_g, const cplx<T>* twz,
-                                                     int P, const TIn* in, const TS* tangent,
+                                                     int P, In in, const work_t<S>* tangent,
                                                      Vec6 a, double sgn,
-                                                     const MinresState* st, cplx<TS>* spec) {
-  if (MODE == kModeMatvec && st->done) return;
+                                                     const MinresState* st, CVec<S> spec) {
+  if constexpr (MODE == kModeMatvec)
+    if (st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + g.nz;
   const int nz = g.nz, nzc = g.nzc;
-  const int M = 3 * P, S = zrow_stride(P), zb = zbuf_elems(P, nz, nzc);
+  const int M = 3 * P, LS = zrow_stride(P), zb = zbuf_elems(P, nz, nzc);
   const long long nlines = (long long)g.nx * g.ny;
   const long long line0 = (long long)blockIdx.x * P;
   load_twiddles(tw, twz, nz);
@@ -115,13 +95,13 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan
     T tau[6] = {T(0), T(0), T(0), T(0), T(0), T(0)};
     if (gl < nlines) {
       const long long p = gl * nz + z;
-      if (MODE == kModeProject) {
+      if constexpr (MODE == kModeProject) {
 #pragma unroll
         for (int c = 0; c < 6; ++c) tau[c] = T(a.v[c] + sgn * double(in[c * g.n + p]));
       } else {
         T v[6];
 #pragma unroll
-        for (int c = 0; c < 6; ++c) v[c] = s * T(in[blk_idx(c, p, 6)]);
+        for (int c = 0; c < 6; ++c) v[c] = s * in.template get<T>(blk_idx(c, p, 6));
 #pragma unroll
         for (int i2 = 0; i2 < 6; ++i2) {
           T acc = T(0);
@@ -132,10 +112,10 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan
       }
     }
 #pragma unroll
-    for (int q = 0; q < 3; ++q) buf[z * S + li * 3 + q] = {tau[2 * q], tau[2 * q + 1]};
+    for (int q = 0; q < 3; ++q) buf[z * LS + li * 3 + q] = {tau[2 * q], tau[2 * q + 1]};
   }
   __syncthreads();
-  const cplx<T>* res = block_fft<T, kZBT, BIG>(buf, buf + zb, M, S, plan, tw, false);
+  const cplx<T>* res = block_fft<T, kZBT, BIG>(buf, buf + zb, M, LS, plan, tw, false);
 
   const T half = T(0.5);
   for (int i = threadIdx.x; i < P * 3 * nzc; i += kZBT) {
@@ -143,20 +123,20 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan
     const int li = m / 3, q = m - 3 * li;
     const long long gl = line0 + li;
     if (gl >= nlines) continue;
-    const cplx<T> zf = res[kz * S + m];
-    const cplx<T> zr = cconj(res[((nz - kz) % nz) * S + m]);
+    const cplx<T> zf = res[kz * LS + m];
+    const cplx<T> zr = cconj(res[((nz - kz) % nz) * LS + m]);
     const cplx<T> xa = {half * (zf.x + zr.x), half * (zf.y + zr.y)};
     const cplx<T> xb = {half * (zf.y - zr.y), -half * (zf.x - zr.x)};
-    spec[(2 * q) * g.nspec + gl * nzc + kz] = cvt<TS>(xa);
-    spec[(2 * q + 1) * g.nspec + gl * nzc + kz] = cvt<TS>(xb);
+    spec.put((2 * q) * g.nspec + gl * nzc + kz, xa);
+    spec.put((2 * q + 1) * g.nspec + gl * nzc + kz, xb);
   }
 }
 
 // ----------------------------------------------------------------- y pass
-template <typename T, typename TS, bool BIG>
+template <typename T, typename S, bool BIG>
 __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g, const cplx<T>* twy, int TY,
                                                   int ntiles, bool inv,
-                                                  const MinresState* st, cplx<TS>* spec) {
+                                                  const MinresState* st, CVec<S> spec) {
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
@@ -168,41 +148,41 @@ __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g,
   const int kz0 = tile * TY;
   const int width = min(TY, g.nzc - kz0);
   const int ny = g.ny;
-  cplx<TS>* base = spec + (long long)cx * ny * g.nzc + kz0;  // spec[c][x][.][kz0]
+  const CVec<S> base = spec.at((long long)cx * ny * g.nzc + kz0);  // spec[c][x][.][kz0]
   load_twiddles(tw, twy, ny);
   // FP32-stored tiles of whole, even-aligned columns move two complex values
   // (16 B) per access (even ny keeps the LDS tile after the ny twiddles 16 B
   // aligned)
-  const bool pairs = sizeof(TS) == 4 && TY % 2 == 0 && width == TY && g.nzc % 2 == 0 && ny % 2 == 0;
+  const bool pairs = CVec<S>::kPairs && TY % 2 == 0 && width == TY && g.nzc % 2 == 0 && ny % 2 == 0;
   const int hp = TY / 2;
-  if constexpr (sizeof(TS) == 4) {
+  if constexpr (CVec<S>::kPairs) {
     if (pairs) {
       for (int i = threadIdx.x; i < hp * ny; i += kYBT) {
         const int y = i / hp, t = 2 * (i - y * hp);
-        load_pair<T>(buf + y * TY + t, base + (long long)y * g.nzc + t);
+        base.get_pair((long long)y * g.nzc + t, buf + y * TY + t);
       }
     }
   }
   if (!pairs) {
     for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
       const int y = i / TY, t = i - y * TY;
-      buf[y * TY + t] = t < width ? cvt<T>(base[(long long)y * g.nzc + t]) : cplx<T>{T(0), T(0)};
+      buf[y * TY + t] = t < width ? base.template get<T>((long long)y * g.nzc + t) : cplx<T>{T(0), T(0)};
     }
   }
   __syncthreads();
   const cplx<T>* res = block_fft<T, kYBT, BIG>(buf, buf + TY * ny, TY, TY, plan, tw, inv);
-  if constexpr (sizeof(TS) == 4) {
+  if constexpr (CVec<S>::kPairs) {
     if (pairs) {
       for (int i = threadIdx.x; i < hp * ny; i += kYBT) {
         const int y = i / hp, t = 2 * (i - y * hp);
-        store_pair<T>(base + (long long)y * g.nzc + t, res + y * TY + t);
+        base.put_pair((long long)y * g.nzc + t, res + y * TY + t);
       }
     }
   }
   if (!pairs) {
     for (int i = threadIdx.x; i < TY * ny; i += kYBT) {
       const int y = i / TY, t = i - y * TY;
-      if (t < width) base[(long long)y * g.nzc + t] = cvt<TS>(res[y * TY + t]);
+      if (t < width) base.put((long long)y * g.nzc + t, res[y * TY + t]);
     }
   }
 }
@@ -244,11 +224,11 @@ __device__ __forceinline__ double rnorm3<double>(double a, double b, double c) {
 // LDS: forward x-FFT of the 6 TX interleaved lines, Ghat on each point, inverse
 // x-FFT, one HBM read and one write of the tile.  Used where TX >= 2 fits the
 // budget (grids up to about 128 points along x).
-template <typename T, typename TS, bool BIG>
+template <typename T, typename S, bool BIG>
 __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
                                                   const T* tanz, int maskbits,
-                                                  const MinresState* st, cplx<TS>* spec) {
+                                                  const MinresState* st, CVec<S> spec) {
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
@@ -261,21 +241,21 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   const int width = min(TX, g.nzc - kz0);
   const int nx = g.nx;
   const long long xstride = (long long)g.ny * g.nzc;
-  cplx<TS>* base = spec + (long long)y * g.nzc + kz0;  // spec[c][0][y][kz0]
+  const CVec<S> base = spec.at((long long)y * g.nzc + kz0);  // spec[c][0][y][kz0]
   load_twiddles(tw, twx, nx);
   const int tot = 6 * nx * TX;
   // FP32-stored tiles of whole, even-aligned columns move two complex values
   // (16 B) per access (even nx keeps the LDS tile after the nx twiddles 16 B
   // aligned)
-  const bool pairs = sizeof(TS) == 4 && TX % 2 == 0 && width == TX && g.nzc % 2 == 0 && nx % 2 == 0;
+  const bool pairs = CVec<S>::kPairs && TX % 2 == 0 && width == TX && g.nzc % 2 == 0 && nx % 2 == 0;
   const int hp = TX / 2;
-  if constexpr (sizeof(TS) == 4) {
+  if constexpr (CVec<S>::kPairs) {
     if (pairs) {
       for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
         const int cxp = i / hp;  // c*nx + x
         const int t = 2 * (i - cxp * hp);
         const int c = cxp / nx, x = cxp - c * nx;
-        load_pair<T>(buf + x * (6 * TX) + c * TX + t, base + c * g.nspec + x * xstride + t);
+        base.get_pair(c * g.nspec + x * xstride + t, buf + x * (6 * TX) + c * TX + t);
       }
     }
   }
@@ -285,7 +265,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
       const int cxp = i / TX;  // c*nx + x
       const int c = cxp / nx, x = cxp - c * nx;
       buf[x * (6 * TX) + c * TX + t] =
-          t < width ? cvt<T>(base[c * g.nspec + x * xstride + t]) : cplx<T>{T(0), T(0)};
+          t < width ? base.template get<T>(c * g.nspec + x * xstride + t) : cplx<T>{T(0), T(0)};
     }
   }
   __syncthreads();
@@ -319,13 +299,13 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   __syncthreads();
   const cplx<T>* out =
       block_fft<T, kXBT, BIG>(res, res == buf ? buf2 : buf, 6 * TX, 6 * TX, plan, tw, true);
-  if constexpr (sizeof(TS) == 4) {
+  if constexpr (CVec<S>::kPairs) {
     if (pairs) {
       for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
         const int cxp = i / hp;
         const int t = 2 * (i - cxp * hp);
         const int c = cxp / nx, x = cxp - c * nx;
-        store_pair<T>(base + c * g.nspec + x * xstride + t, out + x * (6 * TX) + c * TX + t);
+        base.put_pair(c * g.nspec + x * xstride + t, out + x * (6 * TX) + c * TX + t);
       }
     }
   }
@@ -334,7 +314,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
       const int t = i % TX;
       const int cxp = i / TX;
       const int c = cxp / nx, x = cxp - c * nx;
-      if (t < width) base[c * g.nspec + x * xstride + t] = cvt<TS>(out[x * (6 * TX) + c * TX + t]);
+      if (t < width) base.put(c * g.nspec + x * xstride + t, out[x * (6 * TX) + c * TX + t]);
     }
   }
 }
@@ -350,35 +330,35 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
 // four workgroups per CU.
 // With PAIRS (FP32, even TX dividing nzc, even nx) a thread moves two
 // adjacent complex values (16 B) per access and the LDS rows have stride TX.
-template <typename T, typename TS, bool PAIRS, bool BIG>
+template <typename T, typename S, bool PAIRS, bool BIG>
 __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
                                                   const T* tanz, int maskbits,
-                                                  const MinresState* st, cplx<TS>* spec) {
-  static_assert(!PAIRS || (sizeof(T) == 4 && sizeof(TS) == 4),
+                                                  const MinresState* st, CVec<S> spec) {
+  static_assert(!PAIRS || (sizeof(T) == 4 && std::is_same<S, float>::value),
                 "pairs of complex values are 16 B in FP32 only");
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
   const int nx = g.nx;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
-  const int S = PAIRS ? TX : odd_stride(TX);
+  const int LS = PAIRS ? TX : odd_stride(TX);
   cplx<T>* buf = tw + nx;
-  cplx<T>* buf2 = buf + nx * S;
+  cplx<T>* buf2 = buf + nx * LS;
   const int b = xcd_logical_block(blockIdx.x, gridDim.x);
   const int tile = b % ntiles;
   const int y = b / ntiles;
   const int kz0 = tile * TX;
   const int width = min(TX, g.nzc - kz0);
   const long long xstride = (long long)g.ny * g.nzc;
-  cplx<TS>* base = spec + (long long)y * g.nzc + kz0;  // spec[c][0][y][kz0]
+  const CVec<S> base = spec.at((long long)y * g.nzc + kz0);  // spec[c][0][y][kz0]
   load_twiddles(tw, twx, nx);
   // per-thread units of one component tile (a complex value, or a pair of
   // adjacent ones with PAIRS): (x, t) with t fastest
   constexpr int W = PAIRS ? 2 : 1;        // complex values per unit
   constexpr int K = kXTileK / W;          // units per thread
   using Unit = typename std::conditional<PAIRS, float4, cplx<T>>::type;   // in LDS
-  using GUnit = typename std::conditional<PAIRS, float4, cplx<TS>>::type;  // in HBM
+  using GUnit = typename std::conditional<PAIRS, float4, typename CVec<S>::Unit>::type;  // loaded
   const int TU = TX / W;                  // units per row
   const int E = nx * TU;
   int goff[K], loff[K];
@@ -388,20 +368,23 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan
     const int i = threadIdx.x + k * kXBT;
     const int x = i / TU, t = W * (i - x * TU);
     goff[k] = x * int(xstride) + t;
-    loff[k] = x * S + t;
+    loff[k] = x * LS + t;
     on[k] = i < E && t < width;
   }
   GUnit pre[K];
   auto fetch = [&](int c) {
-    const GUnit* src = reinterpret_cast<const GUnit*>(base + c * g.nspec);
 #pragma unroll
-    for (int k = 0; k < K; ++k) pre[k] = on[k] ? src[goff[k] / W] : GUnit{};
+    for (int k = 0; k < K; ++k) {
+      if constexpr (PAIRS) {
+        const float4* src = reinterpret_cast<const float4*>(base.p + c * g.nspec);
+        pre[k] = on[k] ? src[goff[k] / W] : GUnit{};
+      } else {
+        pre[k] = on[k] ? base.load(c * g.nspec + goff[k]) : GUnit{};
+      }
+    }
   };
   auto to_lds = [](const GUnit& u) -> Unit {
-    if constexpr (PAIRS) return u; else return cvt<T>(u);
-  };
-  auto to_hbm = [](const Unit& u) -> GUnit {
-    if constexpr (PAIRS) return u; else return cvt<TS>(u);
+    if constexpr (PAIRS) return u; else return CVec<S>::template value<T>(u);
   };
   auto transform = [&](bool inv) {
     fetch(0);
@@ -411,11 +394,16 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan
         if (threadIdx.x + k * kXBT < E) reinterpret_cast<Unit*>(buf)[loff[k] / W] = to_lds(pre[k]);
       lds_barrier();
       if (c < 5) fetch(c + 1);
-      const cplx<T>* res = block_fft<T, kXBT, BIG>(buf, buf2, TX, S, plan, tw, inv);
-      GUnit* dst = reinterpret_cast<GUnit*>(base + c * g.nspec);
+      const cplx<T>* res = block_fft<T, kXBT, BIG>(buf, buf2, TX, LS, plan, tw, inv);
 #pragma unroll
-      for (int k = 0; k < K; ++k)
-        if (on[k]) dst[goff[k] / W] = to_hbm(reinterpret_cast<const Unit*>(res)[loff[k] / W]);
+      for (int k = 0; k < K; ++k) {
+        if (!on[k]) continue;
+        if constexpr (PAIRS)
+          reinterpret_cast<float4*>(base.p + c * g.nspec)[goff[k] / W] =
+              reinterpret_cast<const float4*>(res)[loff[k] / W];
+        else
+          base.put(c * g.nspec + goff[k], res[loff[k]]);
+      }
       lds_barrier();
     }
   };
@@ -429,10 +417,10 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan
     const int t = i % TX, kx = i / TX;
     const int kz = kz0 + t;
     if (t >= width) continue;
-    cplx<TS>* pt = base + kx * xstride + t;
+    const long long pt = kx * xstride + t;
     cplx<T> X[6];
 #pragma unroll
-    for (int c = 0; c < 6; ++c) X[c] = cvt<T>(pt[c * g.nspec]);
+    for (int c = 0; c < 6; ++c) X[c] = base.template get<T>(pt + c * g.nspec);
     if (kx == nyqx || y == nyqy || kz == nyqz) {
 #pragma unroll
       for (int c = 0; c < 6; ++c) X[c] = {T(0), T(0)};
@@ -445,26 +433,27 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan
       apply_ghat<T>(X, tx * inv, ty * inv, tz * inv);
     }
 #pragma unroll
-    for (int c = 0; c < 6; ++c) pt[c * g.nspec] = cvt<TS>(X[c]);
+    for (int c = 0; c < 6; ++c) base.put(pt + c * g.nspec, X[c]);
   }
   __syncthreads();
   transform(true);
 }
 
 // ----------------------------------------------------------------- z inverse
-template <typename T, typename TS, int MODE, typename TOut, bool BIG>
+template <typename T, typename S, int MODE, bool BIG>
 __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan_g, const cplx<T>* twz,
-                                                     int P, const cplx<TS>* spec, TOut* out,
-                                                     TS* r1, const TS* r2,
+                                                     int P, CVec<S> spec, double* out,
+                                                     RVec<S> r1, RVec<S> r2,
                                                      const MinresState* st, double* partials) {
-  if (MODE == kModeMatvec && st->done) return;
+  if constexpr (MODE == kModeMatvec)
+    if (st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
   cplx<T>* buf = tw + g.nz;
   __shared__ double red[kZBT / 64];
   const int nz = g.nz, nzc = g.nzc;
-  const int M = 3 * P, S = zrow_stride(P), zb = zbuf_elems(P, nz, nzc);
+  const int M = 3 * P, LS = zrow_stride(P), zb = zbuf_elems(P, nz, nzc);
   const long long nlines = (long long)g.nx * g.ny;
   const long long line0 = (long long)blockIdx.x * P;
   load_twiddles(tw, twz, nz);
@@ -474,7 +463,7 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan
   const int run = P * nzc;
   const long long valid = (nlines - line0 < P ? nlines - line0 : P) * nzc;
   bool staged = false;
-  if constexpr (sizeof(TS) == 4) {
+  if constexpr (CVec<S>::kPairs) {
     if (nzc % 2 == 0 && nz % 2 == 0) {
       // FP32 storage, even nzc: two adjacent values per access (16 B; even nz
       // keeps the LDS stage 16 B aligned)
@@ -482,7 +471,7 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan
       for (int i = threadIdx.x; i < 6 * hr; i += kZBT) {
         const int c = i / hr, r = 2 * (i - c * hr);
         if (r < valid) {
-          load_pair<T>(stage + c * run + r, spec + c * g.nspec + line0 * nzc + r);
+          spec.get_pair(c * g.nspec + line0 * nzc + r, stage + c * run + r);
         } else {
           stage[c * run + r] = {T(0), T(0)};
           stage[c * run + r + 1] = {T(0), T(0)};
@@ -494,7 +483,7 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan
   if (!staged) {
     for (int i = threadIdx.x; i < 6 * run; i += kZBT) {
       const int c = i / run, r = i - c * run;
-      stage[i] = r < valid ? cvt<T>(spec[c * g.nspec + line0 * nzc + r]) : cplx<T>{T(0), T(0)};
+      stage[i] = r < valid ? spec.template get<T>(c * g.nspec + line0 * nzc + r) : cplx<T>{T(0), T(0)};
     }
   }
   __syncthreads();
@@ -515,15 +504,15 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan
         zv = {xa.x - xb.y, xa.y + xb.x};
       }
     }
-    buf[k * S + li * 3 + q] = zv;
+    buf[k * LS + li * 3 + q] = zv;
   }
   __syncthreads();
-  const cplx<T>* res = block_fft<T, kZBT, BIG>(buf, buf + zb, M, S, plan, tw, true);
+  const cplx<T>* res = block_fft<T, kZBT, BIG>(buf, buf + zb, M, LS, plan, tw, true);
 
   const T invn = T(1.0 / double(g.n));
   double acc = 0.0;
   T s = T(1), c1 = T(0);
-  if (MODE == kModeMatvec) {
+  if constexpr (MODE == kModeMatvec) {
     s = T(st->s);
     c1 = T(st->c_r1);
   }
@@ -534,22 +523,23 @@ __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan
     const long long p = gl * nz + z;
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const cplx<T> zv = res[z * S + li * 3 + q];
+      const cplx<T> zv = res[z * LS + li * 3 + q];
       const T va = zv.x * invn, vb = zv.y * invn;
-      if (MODE == kModeProject) {
-        out[(2 * q) * g.n + p] = TOut(va);
-        out[(2 * q + 1) * g.n + p] = TOut(vb);
+      if constexpr (MODE == kModeProject) {
+        out[(2 * q) * g.n + p] = double(va);
+        out[(2 * q + 1) * g.n + p] = double(vb);
       } else {
         const long long ia = blk_idx(2 * q, p, 6), ib = blk_idx(2 * q + 1, p, 6);
-        const TS ua = TS(va - c1 * T(r1[ia]));
-        const TS ub = TS(vb - c1 * T(r1[ib]));
-        r1[ia] = ua;
-        r1[ib] = ub;
-        acc += double(s * T(r2[ia])) * double(ua) + double(s * T(r2[ib])) * double(ub);
+        const T ua = va - c1 * r1.template get<T>(ia);
+        const T ub = vb - c1 * r1.template get<T>(ib);
+        r1.put(ia, ua);
+        r1.put(ib, ub);
+        acc += double(s * r2.template get<T>(ia)) * double(ua) +
+               double(s * r2.template get<T>(ib)) * double(ub);
       }
     }
   }
-  if (MODE == kModeMatvec) {
+  if constexpr (MODE == kModeMatvec) {
     const double tot = block_sum<kZBT>(acc, red);
     if (threadIdx.x == 0) partials[blockIdx.x] = tot;
   }
@@ -607,8 +597,8 @@ void allow_lds(K kernel, size_t bytes) {
 
 }  // namespace
 
-template <typename T, typename A>
-Spectral<T, A>::Spectral(int nx, int ny, int nz) {
+template <typename S, typename A>
+void Spectral<S, A>::plan(int nx, int ny, int nz) {
   if (nx < 1 || ny < 1 || nz < 1) throw Error("grid dimensions must be positive");
   g_.nx = nx;
   g_.ny = ny;
@@ -627,7 +617,8 @@ Spectral<T, A>::Spectral(int nx, int ny, int nz) {
   // FP32 data with FP64 arithmetic: the z passes do best with 1.5x the FP64
   // budget (measured at 128^3: z_forward 20.8 -> 19.7 us, z_inverse 24.5 -> 23.0 us;
   // the y and x passes are slower with larger tiles)
-  const size_t zscale = (sizeof(A) == 8 && sizeof(T) == 4) ? 3 : scale;
+  constexpr bool kNarrow = !std::is_same<S, double>::value;  // storage narrower than FP64
+  const size_t zscale = (sizeof(A) == 8 && kNarrow) ? 3 : scale;
   const size_t zbudget = kZBufBytes * zscale, ybudget = kYBufBytes * scale,
                xbudget = kXBufBytes * scale;
   // z passes: P grid lines (3P interleaved complex lines) per workgroup
@@ -660,9 +651,16 @@ Spectral<T, A>::Spectral(int nx, int ny, int nz) {
     if (tx_ == 0) throw Error("nx too large for the x pass");
     tx_ = std::min(tx_, g_.nzc);
     // FP32 with whole, even-width tiles: pairs of complex values per access
-    x_pairs_ = sizeof(T) == 4 && sizeof(A) == 4 && tx_ % 2 == 0 && g_.nzc % tx_ == 0 && nx % 2 == 0;
+    x_pairs_ = std::is_same<S, float>::value && sizeof(A) == 4 && tx_ % 2 == 0 &&
+               g_.nzc % tx_ == 0 && nx % 2 == 0;
   }
-  MTL_HIP(hipMalloc(&spec_, 6 * g_.nspec * sizeof(cplx<T>)));
+}
+
+template <typename S, typename A>
+Spectral<S, A>::Spectral(int nx, int ny, int nz) {
+  plan(nx, ny, nz);
+  MTL_HIP(hipMalloc(&spec_mem_, cplx_bytes<S>(6 * g_.nspec)));
+  spec_ = cvec_over<S>(spec_mem_, 6 * g_.nspec);
   twx_ = upload(twiddle_table<A>(nx));
   twy_ = upload(twiddle_table<A>(ny));
   twz_ = upload(twiddle_table<A>(nz));
@@ -674,27 +672,95 @@ Spectral<T, A>::Spectral(int nx, int ny, int nz) {
   tanz_ = upload(tan_table<A>(nz, g_.nzc));
 }
 
-template <typename T, typename A>
-Spectral<T, A>::~Spectral() {
-  for (void* p : {(void*)spec_, (void*)twx_, (void*)twy_, (void*)twz_, (void*)tanx_,
+namespace {
+// bytes of each buffer the spectral passes read, in workspace order, each
+// rounded up to 256 B
+template <typename S, typename A>
+void workspace_layout(int nx, int ny, int nz, size_t sz[10]) {
+  const long long nzc = (nz + 1) / 2;
+  const long long nspec = (long long)nx * ny * nzc;
+  const size_t raw[10] = {cplx_bytes<S>(6 * nspec),
+                          size_t(nx) * sizeof(cplx<A>), size_t(ny) * sizeof(cplx<A>),
+                          size_t(nz) * sizeof(cplx<A>), sizeof(AxisPlan), sizeof(AxisPlan),
+                          sizeof(AxisPlan), size_t(nx) * sizeof(A), size_t(ny) * sizeof(A),
+                          size_t(nzc) * sizeof(A)};
+  for (int i = 0; i < 10; ++i) sz[i] = (raw[i] + 255) / 256 * 256;
+}
+}  // namespace
+
+template <typename S, typename A>
+size_t Spectral<S, A>::workspace_bytes(int nx, int ny, int nz) {
+  size_t sz[10], tot = 0;
+  workspace_layout<S, A>(nx, ny, nz, sz);
+  for (size_t v : sz) tot += v;
+  return tot;
+}
+
+template <typename S, typename A>
+Spectral<S, A>::Spectral(int nx, int ny, int nz, void* workspace, size_t bytes, hipStream_t s) {
+  plan(nx, ny, nz);
+  external_ = true;
+  if (workspace == nullptr || (reinterpret_cast<uintptr_t>(workspace) & 255) != 0)
+    throw Error("spectral workspace must be a 256-byte aligned device address");
+  if (bytes < workspace_bytes(nx, ny, nz)) throw Error("spectral workspace too small");
+  size_t sz[10];
+  workspace_layout<S, A>(nx, ny, nz, sz);
+  unsigned char* w = static_cast<unsigned char*>(workspace);
+  void* slot[10];
+  for (int i = 0; i < 10; ++i) {
+    slot[i] = w;
+    w += sz[i];
+  }
+  spec_ = cvec_over<S>(slot[0], 6 * g_.nspec);
+  twx_ = static_cast<cplx<A>*>(slot[1]);
+  twy_ = static_cast<cplx<A>*>(slot[2]);
+  twz_ = static_cast<cplx<A>*>(slot[3]);
+  pxd_ = static_cast<AxisPlan*>(slot[4]);
+  pyd_ = static_cast<AxisPlan*>(slot[5]);
+  pzd_ = static_cast<AxisPlan*>(slot[6]);
+  tanx_ = static_cast<A*>(slot[7]);
+  tany_ = static_cast<A*>(slot[8]);
+  tanz_ = static_cast<A*>(slot[9]);
+  // the host tables live until the stream has copied them
+  const auto tx = twiddle_table<A>(nx), ty = twiddle_table<A>(ny), tz = twiddle_table<A>(nz);
+  const auto ax = tan_table<A>(nx, nx), ay = tan_table<A>(ny, ny), az = tan_table<A>(nz, g_.nzc);
+  auto put = [&](void* dst, const void* src, size_t n) {
+    MTL_HIP(hipMemcpyAsync(dst, src, n, hipMemcpyHostToDevice, s));
+  };
+  put(twx_, tx.data(), tx.size() * sizeof(cplx<A>));
+  put(twy_, ty.data(), ty.size() * sizeof(cplx<A>));
+  put(twz_, tz.data(), tz.size() * sizeof(cplx<A>));
+  put(pxd_, &px_, sizeof(AxisPlan));
+  put(pyd_, &py_, sizeof(AxisPlan));
+  put(pzd_, &pz_, sizeof(AxisPlan));
+  put(tanx_, ax.data(), ax.size() * sizeof(A));
+  put(tany_, ay.data(), ay.size() * sizeof(A));
+  put(tanz_, az.data(), az.size() * sizeof(A));
+  stream_sync(s);
+}
+
+template <typename S, typename A>
+Spectral<S, A>::~Spectral() {
+  if (external_) return;
+  for (void* p : {spec_mem_, (void*)twx_, (void*)twy_, (void*)twz_, (void*)tanx_,
                   (void*)tany_, (void*)tanz_, (void*)pxd_, (void*)pyd_, (void*)pzd_})
     if (p) (void)hipFree(p);
 }
 
-template <typename T, typename A>
-int Spectral<T, A>::zinv_blocks() const {
+template <typename S, typename A>
+int Spectral<S, A>::zinv_blocks() const {
   return ceil_div((long long)g_.nx * g_.ny, pz_lines_);
 }
 
-template <typename T, typename A>
-template <int MODE, typename TIn, typename TOut>
-void Spectral<T, A>::run(const TIn* in, const T* tangent, TOut* out, T* r1, const Vec6& a, double sgn,
-                      const Vec6& mask, const MinresState* st, double* partials,
-                      hipStream_t s) {
+template <typename S, typename A>
+template <int MODE, typename In>
+void Spectral<S, A>::run(In in, const W* tangent, double* out, RVec<S> r1, const Vec6& a,
+                         double sgn, const Vec6& mask, const MinresState* st, double* partials,
+                         hipStream_t s) {
   const int P = pz_lines_;
   const int zblocks = zinv_blocks();
   const size_t zlds = (size_t(g_.nz) + 2 * size_t(zbuf_elems(P, g_.nz, g_.nzc))) * sizeof(cplx<A>);
-  auto kzf = big_z_ ? k_z_forward<A, T, MODE, TIn, true> : k_z_forward<A, T, MODE, TIn, false>;
+  auto kzf = big_z_ ? k_z_forward<A, S, MODE, In, true> : k_z_forward<A, S, MODE, In, false>;
   allow_lds(kzf, zlds);
   profiler().launch(MODE == kModeMatvec ? kKZForward : residual_id(kKZForward), kzf, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, in, tangent, a, sgn, st, spec_);
   MTL_LAUNCH_CHECK();
@@ -702,7 +768,7 @@ void Spectral<T, A>::run(const TIn* in, const T* tangent, TOut* out, T* r1, cons
   const MinresState* stp = MODE == kModeMatvec ? st : nullptr;
   const int ytiles = ceil_div(g_.nzc, ty_);
   const size_t ylds = (size_t(g_.ny) + 2 * size_t(ty_) * g_.ny) * sizeof(cplx<A>);
-  auto ky = big_y_ ? k_y_pass<A, T, true> : k_y_pass<A, T, false>;
+  auto ky = big_y_ ? k_y_pass<A, S, true> : k_y_pass<A, S, false>;
   allow_lds(ky, ylds);
   profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, false, stp, spec_);
   MTL_LAUNCH_CHECK();
@@ -712,15 +778,15 @@ void Spectral<T, A>::run(const TIn* in, const T* tangent, TOut* out, T* r1, cons
       x_resident_ ? (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<A>)
       : x_pairs_  ? (size_t(g_.nx) + 2 * size_t(tx_) * g_.nx) * sizeof(cplx<A>)
                   : x_lds_bytes(g_.nx, tx_, sizeof(cplx<A>));
-  auto kx = big_x_ ? k_x_ghat_resident<A, T, true> : k_x_ghat_resident<A, T, false>;
+  auto kx = big_x_ ? k_x_ghat_resident<A, S, true> : k_x_ghat_resident<A, S, false>;
   if (!x_resident_) {
-    if constexpr (sizeof(T) == 4 && sizeof(A) == 4) {
+    if constexpr (std::is_same<S, float>::value && sizeof(A) == 4) {
       if (x_pairs_)
-        kx = big_x_ ? k_x_ghat_streamed<A, T, true, true> : k_x_ghat_streamed<A, T, true, false>;
+        kx = big_x_ ? k_x_ghat_streamed<A, S, true, true> : k_x_ghat_streamed<A, S, true, false>;
       else
-        kx = big_x_ ? k_x_ghat_streamed<A, T, false, true> : k_x_ghat_streamed<A, T, false, false>;
+        kx = big_x_ ? k_x_ghat_streamed<A, S, false, true> : k_x_ghat_streamed<A, S, false, false>;
     } else {
-      kx = big_x_ ? k_x_ghat_streamed<A, T, false, true> : k_x_ghat_streamed<A, T, false, false>;
+      kx = big_x_ ? k_x_ghat_streamed<A, S, false, true> : k_x_ghat_streamed<A, S, false, false>;
     }
   }
   allow_lds(kx, xlds);
@@ -730,39 +796,30 @@ void Spectral<T, A>::run(const TIn* in, const T* tangent, TOut* out, T* r1, cons
   profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, true, stp, spec_);
   MTL_LAUNCH_CHECK();
 
-  auto kzi = big_z_ ? k_z_inverse<A, T, MODE, TOut, true> : k_z_inverse<A, T, MODE, TOut, false>;
+  RVec<S> r2{};
+  if constexpr (MODE == kModeMatvec) r2 = in;
+  auto kzi = big_z_ ? k_z_inverse<A, S, MODE, true> : k_z_inverse<A, S, MODE, false>;
   allow_lds(kzi, zlds);
-  profiler().launch(MODE == kModeMatvec ? kKZInverse : residual_id(kKZInverse), kzi, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, spec_, out, r1, reinterpret_cast<const T*>(in), st, partials);
+  profiler().launch(MODE == kModeMatvec ? kKZInverse : residual_id(kKZInverse), kzi, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, spec_, out, r1, r2, st, partials);
   MTL_LAUNCH_CHECK();
 }
 
-template <typename T, typename A>
-template <typename TIn, typename TOut>
-void Spectral<T, A>::project(const TIn* in, TOut* out, const Vec6& a, double sgn, const Vec6& mask,
-                          hipStream_t s) {
-  run<kModeProject, TIn, TOut>(in, nullptr, out, nullptr, a, sgn, mask, nullptr, nullptr, s);
+template <typename S, typename A>
+void Spectral<S, A>::project(const double* in, double* out, const Vec6& a, double sgn,
+                             const Vec6& mask, hipStream_t s) {
+  run<kModeProject, const double*>(in, nullptr, out, RVec<S>{}, a, sgn, mask, nullptr, nullptr, s);
 }
 
-template <typename T, typename A>
-int Spectral<T, A>::matvec_minres(const T* r2, const T* tangent, T* r1, const Vec6& mask,
-                               const MinresState* st, double* partials, hipStream_t s) {
+template <typename S, typename A>
+int Spectral<S, A>::matvec_minres(RVec<S> r2, const W* tangent, RVec<S> r1, const Vec6& mask,
+                                  const MinresState* st, double* partials, hipStream_t s) {
   Vec6 zero{};
-  run<kModeMatvec, T, T>(r2, tangent, static_cast<T*>(nullptr), r1, zero, 1.0, mask, st,
-                         partials, s);
+  run<kModeMatvec, RVec<S>>(r2, tangent, nullptr, r1, zero, 1.0, mask, st, partials, s);
   return zinv_blocks();
 }
 
 template class Spectral<float>;
 template class Spectral<double>;
-template class Spectral<float, double>;
-template void Spectral<double>::project<double, double>(const double*, double*, const Vec6&,
-                                                        double, const Vec6&, hipStream_t);
-template void Spectral<float>::project<double, float>(const double*, float*, const Vec6&, double,
-                                                      const Vec6&, hipStream_t);
-template void Spectral<float>::project<double, double>(const double*, double*, const Vec6&,
-                                                       double, const Vec6&, hipStream_t);
-template void Spectral<float, double>::project<double, double>(const double*, double*,
-                                                               const Vec6&, double, const Vec6&,
-                                                               hipStream_t);
+template class Spectral<f48, double>;
 
 }  // namespace mtl

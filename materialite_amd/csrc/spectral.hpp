@@ -4,6 +4,7 @@
 
 #include "fft.hpp"
 #include "krylov.hpp"
+#include "storage.hpp"
 
 namespace mtl {
 
@@ -16,36 +17,43 @@ __host__ __device__ constexpr int sym6(int i, int j) {
   return i <= j ? i * 6 - (i * (i - 1)) / 2 + (j - i) : j * 6 - (j * (j - 1)) / 2 + (i - j);
 }
 
-// T: type of the spectrum and of the Krylov vectors and tangent in HBM;
-// A: arithmetic type of the FFT passes (LDS tiles, twiddles, Ghat).  A = double
-// with T = float transforms FP32 data with FP64 butterflies.
-template <typename T, typename A = T>
+// S: storage of the spectrum and of the Lanczos vectors in HBM (float,
+// double or f48); the tangent and the MINRES w / x vectors are work_t<S>.
+// A: arithmetic type of the FFT passes (LDS tiles, twiddles, Ghat).
+template <typename S, typename A = S>
 class Spectral {
  public:
+  using W = work_t<S>;
   Spectral(int nx, int ny, int nz);
+  // The same, with the spectrum, twiddle, tan and plan tables carved out of a
+  // caller-provided device workspace of `bytes` bytes: nothing is allocated on
+  // the device; the tables are copied to it on stream s before returning.
+  Spectral(int nx, int ny, int nz, void* workspace, size_t bytes, hipStream_t s);
+  // device bytes the workspace constructor needs for this grid
+  static size_t workspace_bytes(int nx, int ny, int nz);
   ~Spectral();
   Spectral(const Spectral&) = delete;
   Spectral& operator=(const Spectral&) = delete;
 
   const Grid& grid() const { return g_; }
 
-  // out = F^-1[ Ghat . F(a + sgn*in) ] (real part), fields SoA [6][n].
-  template <typename TIn, typename TOut>
-  void project(const TIn* in, TOut* out, const Vec6& a, double sgn, const Vec6& mask,
+  // out = F^-1[ Ghat . F(a + sgn*in) ] (real part), fields SoA [6][n] FP64.
+  void project(const double* in, double* out, const Vec6& a, double sgn, const Vec6& mask,
                hipStream_t s);
 
   // One MINRES matvec fused with its vector work:
   //   tau = D : (s * r2);  Av = F^-1 Ghat F tau;  u = Av - c_r1 * r1 -> r1;
   //   partials[b] = sum over the block of (s * r2) . u.
   // Returns the number of partial sums written.
-  int matvec_minres(const T* r2, const T* tangent, T* r1, const Vec6& mask,
+  int matvec_minres(RVec<S> r2, const W* tangent, RVec<S> r1, const Vec6& mask,
                     const MinresState* st, double* partials, hipStream_t s);
 
   int zinv_blocks() const;
 
  private:
-  template <int MODE, typename TIn, typename TOut>
-  void run(const TIn* in, const T* tangent, TOut* out, T* r1, const Vec6& a, double sgn,
+  void plan(int nx, int ny, int nz);
+  template <int MODE, typename In>
+  void run(In in, const W* tangent, double* out, RVec<S> r1, const Vec6& a, double sgn,
            const Vec6& mask, const MinresState* st, double* partials, hipStream_t s);
 
   Grid g_;
@@ -57,13 +65,15 @@ class Spectral {
   bool x_resident_;  // x pass keeps all six components of its tile in LDS
   bool x_pairs_ = false;  // streamed x pass moves pairs of complex values
   bool big_x_ = false, big_y_ = false, big_z_ = false;  // axis plans use radix 11 or 13
-  cplx<T>* spec_ = nullptr;
+  void* spec_mem_ = nullptr;
+  CVec<S> spec_;
   cplx<A>* twx_ = nullptr;
   cplx<A>* twy_ = nullptr;
   cplx<A>* twz_ = nullptr;
   A* tanx_ = nullptr;
   A* tany_ = nullptr;
   A* tanz_ = nullptr;
+  bool external_ = false;  // buffers live in a caller's workspace
 };
 
 }  // namespace mtl

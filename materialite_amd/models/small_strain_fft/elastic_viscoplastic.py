@@ -2,17 +2,30 @@
 (``small_strain_fft/elastic_viscoplastic.py:20-294``).
 
 The per-point implicit update, its masked iteration order and the consistent
-tangent run in the HIP constitutive kernel; this class carries the parameters
-and names the state variables the solver can return.
+tangent run in the HIP constitutive kernels, inside ``SmallStrainFFT`` on its
+grid handle or through the reference's protocol (``initialize``,
+``calculate_stress_and_tangent``, ``update_state_variables``, ...) on a point
+handle; this class carries the parameters and names the state variables.
 """
 
 import numpy as np
 
 from ...tensor import Order2SymmetricTensor, Scalar, Vector
+from ._device_model import DeviceModel
 from .hardening_laws import KERNEL_LAWS
 
 
-class ElasticViscoplastic:
+class ElasticViscoplastic(DeviceModel):
+    # state held on the device: trial slip resistances and slip shears, the
+    # committed plastic strain in the specimen frame (elastic_viscoplastic.py:
+    # 175-182, 255-271)
+    _device_fields = {
+        "slip_resistances": (Scalar, "ps"),
+        "plastic_strains": (Order2SymmetricTensor, "p"),
+        "slip_system_shear_strains": (Scalar, "ps"),
+        "accumulated_slip": (Scalar, "p"),
+    }
+
     def __init__(self, stiffness, slip_systems, reference_slip_rate, rate_exponent,
                  slip_resistance, hardening_function, hardening_properties):
         if hardening_function not in KERNEL_LAWS:
@@ -36,6 +49,22 @@ class ElasticViscoplastic:
         n = len(slip_systems)
         self._state_variable_info = dict(zip(self.available_state_variables,
                                              [(Scalar, n), (Order2SymmetricTensor, 0), (Scalar, n)]))
+
+    def _add_to(self, native):
+        kind, props = self.kernel_hardening()
+        return native.add_evp(self.stiffness.components, self.slip_systems.schmid_tensor.components,
+                              float(self.reference_slip_rate.components),
+                              float(self.rate_exponent.components),
+                              float(self.initial_slip_resistance), kind, props)
+
+    def initialize(self, orientations):
+        """elastic_viscoplastic.py:52-89: zero plastic state, slip resistances at
+        their initial value; returns the elastic stiffness in the specimen frame."""
+        tangent = super().initialize(orientations)
+        schmid = self.slip_systems.schmid_tensor
+        self.state_variables["stiffnesses"] = self.stiffness
+        self.state_variables["schmid_tensors"] = schmid
+        return tangent
 
     def kernel_hardening(self):
         kind, names = KERNEL_LAWS[self.hardening_function]

@@ -8,10 +8,12 @@ runs on the device through ``mtl_solver_step``; this module keeps the
 reference's host-side control flow and the ``Material`` round trip.
 
 Extra keyword (constructor): ``precision`` = ``"fp64"`` (default; every field,
-FFT and Krylov vector in FP64, reproducing the reference), ``"mixed"`` (FP32
-Krylov vectors, tangents and matvec FFTs; FP64 strain/stress, residual FFT and
-norms) or ``"mixed-fft64"`` (as ``"mixed"``, with FP64 arithmetic inside the
-matvec FFT passes: the data stay FP32 in device memory).
+FFT and Krylov vector in FP64, reproducing the reference), ``"refined"`` (FP32
+tangent and MINRES w / x vectors; matvec spectrum and Lanczos vectors stored as
+an FP32 value plus a bf16 remainder, FFT arithmetic in FP64; FP64 strain/stress,
+residual FFT and norms: follows the FP64 form's iterates to ~1e-8) or
+``"mixed"`` (FP32 Krylov vectors, tangents, matvec FFT storage and arithmetic;
+the fastest form, a few 1e-6 to 1e-5 from the FP64 form).
 """
 
 import logging
@@ -59,6 +61,15 @@ class SmallStrainFFT:
             raise ValueError("must specify phase label or provide a constitutive model")
         n = self._num_points
         if phase_label is None:
+            if isinstance(constitutive_model, Multiphase):
+                # a Multiphase built by the caller (multiphase.py:18-29): its
+                # models on the points of their index lists
+                model_of_point = np.full(n, -1, np.int32)
+                for k, idx in enumerate(constitutive_model.indices):
+                    model_of_point[np.asarray(idx, int)] = k
+                if np.any(model_of_point < 0):
+                    raise ValueError("every point needs a constitutive model")
+                return list(constitutive_model.models), model_of_point
             return [constitutive_model], np.zeros(n, np.int32)
         table = material.extract_regional_field(phase_label)
         phases = table[phase_label].to_list()
@@ -74,23 +85,11 @@ class SmallStrainFFT:
 
     @staticmethod
     def _register(native, model):
-        if isinstance(model, Elastic):
-            return native.add_elastic(model.stiffness.components)
-        if isinstance(model, ElasticViscoplastic):
-            kind, props = model.kernel_hardening()
-            schmid = model.slip_systems.schmid_tensor.components
-            return native.add_evp(model.stiffness.components, schmid,
-                                  float(model.reference_slip_rate.components),
-                                  float(model.rate_exponent.components),
-                                  float(model.initial_slip_resistance), kind, props)
-        if isinstance(model, IsotropicElasticPlastic):
-            kind, props = model.kernel_hardening()
-            return native.add_j2(model.stiffness.components, model.shear_modulus,
-                                 model.bulk_modulus, float(model.yield_stress.components),
-                                 kind, props)
+        if isinstance(model, (Elastic, ElasticViscoplastic, IsotropicElasticPlastic)):
+            return model._add_to(native)
         raise TypeError(
             f"constitutive model {type(model).__name__} has no device implementation "
-            "(supported: Elastic, ElasticViscoplastic, IsotropicElasticPlastic)")
+            "(supported: Elastic, ElasticViscoplastic, IsotropicElasticPlastic, Multiphase of them)")
 
     def _new_time_increment(self, time, time_increment, next_output_time, max_constit_iters):
         """small_strain_fft.py:266-279."""
@@ -164,6 +163,7 @@ class SmallStrainFFT:
             time_step_id = 0
             time_increment = self.initial_time_increment
             krylov = newton = 0
+            floored = False
             while time < (self.end_time - time_tolerance):
                 self._logger.info(f"New increment: time {time} + {time_increment}")
                 d_strain = ls.strain_increment(time, time_increment).components
@@ -183,6 +183,11 @@ class SmallStrainFFT:
                 p.time_step_id = int(time_step_id)
                 p.reset_fluctuation = int(max_strain_increment * old_max_strain_increment < 0.0)
                 rep = native.step(p)
+                if rep.linear_solver_tolerance_used > linear_solver_tolerance and not floored:
+                    floored = True
+                    self._logger.warning(
+                        f"precision {self.precision!r}: linear_solver_tolerance "
+                        f"{linear_solver_tolerance:g} raised to {rep.linear_solver_tolerance_used:g}")
                 krylov += rep.krylov_iterations
                 newton += rep.newton_iterations
                 self._logger.debug(f"global iterations {rep.newton_iterations}, "
@@ -232,7 +237,10 @@ class SmallStrainFFT:
             outputs.update({"stress": Order2SymmetricTensor(native.get_field("stress"), "p"),
                             "strain": Order2SymmetricTensor(native.get_field("strain"), "p")})
             self.stats = {"krylov_iterations": int(krylov), "newton_iterations": int(newton),
-                          "steps": int(time_step_id)}
+                          "steps": int(time_step_id),
+                          "linear_solver_tolerance_used": max(
+                              float(linear_solver_tolerance),
+                              1.0e-5 if self.precision == "mixed" else 0.0)}
         finally:
             native.close()
 

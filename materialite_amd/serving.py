@@ -34,9 +34,11 @@ import numpy as np
 # spectral.hip, minres.hip, constitutive.hip), h = nzc / nz ~ 1/2:
 #   FP64 strain/stress fields and work vectors: 7 x 6 doubles + rotation 9 doubles
 #   FP64 spectrum (residual projection): 6 complex doubles x h
-#   Krylov FP32/FP64 spectrum (mixed only): 6 complex floats x h
-#   MINRES: 6 Mandel vectors of the Krylov type
-#   tangent and committed tangent: 2 x 21 packed of the Krylov type
+#   matvec spectrum (FP32 forms only): 6 complex values x h, 8 bytes each
+#     (mixed) or 12 (refined: FP32 value plus bf16 remainder per real)
+#   MINRES: 2 Lanczos vectors (Mandel-6; 4, 6 or 8 bytes per value) and 4 w / x
+#     vectors of the tangent type
+#   tangent and committed tangent: 2 x 21 packed of the tangent type
 _FIELDS = 7 * 6 * 8 + 9 * 8
 # per point of a model: elastic index; EVP iteration, committed, pending and
 # trial state (EvpModel in csrc/constitutive.hip): per slip system the slip
@@ -55,10 +57,11 @@ def footprint_bytes(dims, precision="mixed", evp_points=0, num_slip_systems=12):
     n = int(np.prod(dims))
     nz = int(dims[2])
     h = ((nz + 1) // 2) / nz
-    mixed = precision in ("mixed", "mixed-fft64")
-    k = 4 if mixed else 8
-    per_voxel = _FIELDS + 6 * 16 * h + (6 * 8 * h if mixed else 0)
-    per_voxel += 6 * 6 * k + 2 * 21 * k + _ELASTIC_POINT
+    k = 8 if precision == "fp64" else 4  # tangent and w / x vectors
+    lanczos = {"fp64": 8, "mixed": 4, "refined": 6}[precision]
+    spec = {"fp64": 0, "mixed": 8, "refined": 12}[precision]
+    per_voxel = _FIELDS + 6 * 16 * h + 6 * spec * h
+    per_voxel += 2 * 6 * lanczos + 4 * 6 * k + 2 * 21 * k + _ELASTIC_POINT
     evp = evp_points * (_EVP_POINT_FIXED + num_slip_systems * _EVP_POINT_PER_SYSTEM)
     # allocation granularity (about 30 device allocations per handle) and
     # per-handle fixed buffers
@@ -225,6 +228,52 @@ def cubic_elastic():
     return Elastic(Order4SymmetricTensor.from_cubic_constants(250e3, 150e3, 120e3))
 
 
+def grain_material(n, grains, seed):
+    """n^3 periodic-Voronoi polycrystal (voxel i at coordinate i, period n) with
+    one Haar-random orientation per grain, both drawn from default_rng(seed)
+    (grain seeds first, then orientations)."""
+    from .material import Material
+    from .microstructure import grain_rotations, periodic_voronoi
+    from .tensor import Orientation
+
+    rng = np.random.default_rng(seed)
+    grain_of_point, _ = periodic_voronoi([n] * 3, grains, rng)
+    rot = grain_rotations(grains, rng)
+    m = Material(dimensions=[n] * 3).create_fields({"grain": grain_of_point})
+    return m.create_regional_fields(
+        "grain", {"grain": np.arange(grains), "orientation": Orientation(rot, "p")})
+
+
+def two_phase_material(n=256, grains=1000, seed=20260005, radius=8.0, volume_fraction=0.10):
+    """The two-phase workload: a periodic-Voronoi FCC matrix (config3_evp) with
+    randomly placed non-overlapping periodic spheres of the given radius up to
+    the volume fraction, elastic and isotropic with bulk and shear moduli 10x the
+    matrix's Voigt averages (phase 1).  Draws grains, orientations, then sphere
+    centres from default_rng(seed).  Returns (material, number of spheres)."""
+    import pandas as pd
+
+    from .material import Material
+    from .microstructure import grain_rotations, periodic_spheres, periodic_voronoi
+    from .models.small_strain_fft import Elastic
+    from .tensor import Order4SymmetricTensor, Orientation
+
+    rng = np.random.default_rng(seed)
+    grain_of_point, _ = periodic_voronoi([n] * 3, grains, rng)
+    rot = grain_rotations(grains, rng)
+    inclusion, centres = periodic_spheres([n] * 3, radius, volume_fraction, rng)
+    c11, c12, c44 = 250e3, 150e3, 120e3
+    k = 10 * (c11 + 2 * c12) / 3
+    mu = 10 * (c11 - c12 + 3 * c44) / 5
+    e = 9 * k * mu / (3 * k + mu)
+    m = Material(dimensions=[n] * 3).create_fields(
+        {"grain": grain_of_point, "phase": inclusion.astype(np.int64)})
+    m = m.create_regional_fields("grain", {"grain": np.arange(grains),
+                                           "orientation": Orientation(rot, "p")})
+    table = pd.DataFrame({"phase": [0, 1], "constitutive_model": [
+        config3_evp(), Elastic(Order4SymmetricTensor.from_isotropic_constants(e, mu))]})
+    return m.create_regional_fields("phase", table), len(centres)
+
+
 @dataclass
 class StreamSpec:
     """One request of the mixed-size stream, drawn from its own seed."""
@@ -246,7 +295,7 @@ def draw_stream_spec(seed, sizes=(32, 48, 64, 96), grains=(50, 500), steps=20, d
     return StreamSpec(seed=seed, n=n, grains=g, model=model, direction=direction, steps=steps, dt=dt)
 
 
-def build_stream_request(spec, precision="mixed", keep_material=False):
+def build_stream_request(spec, precision="refined", keep_material=False):
     """Material (periodic Voronoi, Haar-random orientations from the spec's
     seed), model and schedule of one request; returns a Request whose run()
     performs the solve and returns the volume-averaged stress (Mandel, FP64)

@@ -120,6 +120,10 @@ class SmallStrainFFT:
         newton_total = 0
         krylov_per_solve = []
         steps_done = 0
+        # every attempted increment: (time, dt, converged, Newton iterations,
+        # Krylov iterations, constitutive failure)
+        history = []
+        self.stats = {"history": history}
 
         while time < (self.end_time - time_tol):
             if max_steps is not None and steps_done >= max_steps:
@@ -141,12 +145,15 @@ class SmallStrainFFT:
             converged = False
             iteration = 0
             all_iters = []
+            attempt_krylov = 0
+            failed = False
             while not converged:
                 iteration += 1
                 if iteration == 1:
                     stress, tangent, it = model.calculate_stress_and_tangent(strain, guess, dt)
                     if not it:
                         tangent = old_tangent.copy()
+                        failed = True
                         break
                     b = project(applied - stress)
                     vm = _von_mises_of_mean(stress)
@@ -157,6 +164,7 @@ class SmallStrainFFT:
                 x, itn, _ = minres(lambda v: lhs(v, tangent), b, rtol=linear_solver_tolerance,
                                    maxiter=6 * n_pts * 5)
                 krylov_total += itn
+                attempt_krylov += itn
                 krylov_per_solve.append(itn)
                 newton_total += 1
                 fluct = x.reshape(n_pts, 6)
@@ -166,6 +174,7 @@ class SmallStrainFFT:
                 stress, tangent, it = model.calculate_stress_and_tangent(strain, guess, dt)
                 if not it:
                     tangent = old_tangent.copy()
+                    failed = True
                     break
                 all_iters.append(it)
                 b = project(applied - stress)
@@ -195,6 +204,7 @@ class SmallStrainFFT:
                     self.logger.info("Too many iterations in macroloading loop")
                     break
 
+            history.append((time, dt, converged, iteration, attempt_krylov, failed))
             if converged:
                 model.update_state_variables()
                 old_stress = stress.copy()
@@ -219,6 +229,7 @@ class SmallStrainFFT:
             "newton_iterations": newton_total,
             "krylov_per_solve": krylov_per_solve,
             "steps": step_id,
+            "history": history,
         }
         out = {"stress": stress, "strain": strain}
         out.update(model.outputs() if hasattr(model, "outputs") else {})

@@ -46,7 +46,7 @@ def slip_systems(families):
 
 @pytest.mark.parametrize("name,precision",
                          [(f, "fp64") for f in FAMILIES]
-                         + [(f, p) for f in ("basal_prismatic", "all") for p in ("mixed", "mixed-fft64")])
+                         + [(f, p) for f in ("basal_prismatic", "all") for p in ("mixed", "refined")])
 def test_hcp_evp_fields_match_oracle(name, precision):
     """FP64: fields equal the oracle's to 1e-9.  Mixed forms: stress within the
     1e-5 field tolerance and slip resistances within 1e-4 of the oracle."""

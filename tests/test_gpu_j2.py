@@ -19,7 +19,7 @@ from oracle import constitutive as OC  # noqa: E402
 from oracle import solver as OS  # noqa: E402
 from tests import fixtures as F  # noqa: E402
 
-PRECISIONS = ["fp64", "mixed", "mixed-fft64"]
+PRECISIONS = ["fp64", "mixed", "refined"]
 LAWS = {"perfect_plasticity": perfect_plasticity, "linear": linear, "voce": voce}
 
 

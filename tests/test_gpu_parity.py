@@ -35,7 +35,7 @@ GRIDS = [(3, 4, 5), (4, 4, 4), (2, 3, 6), (1, 4, 3), (6, 5, 8), (8, 8, 8), (7, 9
 
 
 @pytest.mark.parametrize("dims", GRIDS)
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-fft64"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "refined"])
 def test_projection_matches_oracle(dims, precision):
     from materialite_amd._native import NativeSolver
     from oracle.projection import apply_projection
@@ -54,7 +54,8 @@ def test_projection_matches_oracle(dims, precision):
         got = y.cpu().numpy().T
         sol.close()
         scale = np.abs(ref).max()
-        tol = 1e-13 if precision == "fp64" else 2e-6
+        # refined: spectrum kept to 32 significant bits between the passes
+        tol = {"fp64": 1e-13, "refined": 1e-8, "mixed": 2e-6}[precision]
         assert np.abs(got - ref).max() <= tol * scale, (dims, precision, np.abs(got - ref).max() / scale)
 
 
@@ -88,7 +89,7 @@ def test_invalid_model_arguments_are_errors():
 
 
 @pytest.mark.parametrize("dims,precision", [((128, 128, 128), "fp64"), ((128, 128, 128), "mixed"),
-                                             ((128, 128, 128), "mixed-fft64"),
+                                             ((128, 128, 128), "refined"),
                                              ((256, 256, 256), "mixed")])
 def test_projection_properties_full_size(dims, precision):
     """At the workload sizes (128^3, and 256^3 where the x pass streams the
@@ -115,7 +116,7 @@ def test_projection_properties_full_size(dims, precision):
     ppx = proj(px)
     pxw = proj(2.0 * x - 3.0 * w)
     sol.close()
-    tol = 1e-12 if precision == "fp64" else 2e-6
+    tol = {"fp64": 1e-12, "refined": 1e-8, "mixed": 2e-6}[precision]
     scale = px.abs().max().item()
     assert (ppx - px).abs().max().item() <= tol * scale
     assert (pxw - (2.0 * px - 3.0 * pw)).abs().max().item() <= 5 * tol * scale
@@ -133,7 +134,28 @@ def _pore_case():
     return rot, pore, mp
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-fft64"])
+def _check_minres(precision, x, itn, x_ref, itn_ref, lhs, b, bounds):
+    """FP64 and refined: the oracle's iteration count (+-1) and iterate.  Every
+    form: the device iterate against the oracle's iterate after the same number
+    of iterations, within `bounds[precision]` of its largest value (a Krylov
+    bug moves it by order one; rounding by far less: the bounds are a few times
+    the deviations measured on MI355X)."""
+    from oracle.minres import minres
+
+    scale = np.abs(x_ref).max()
+    x_k, itn_k, _ = minres(lhs, b, rtol=1e-30, maxiter=itn)
+    assert itn_k == itn
+    dev_k = np.abs(x - x_k).max() / np.abs(x_k).max()
+    print(f"iterations {itn} (oracle {itn_ref}); iterate vs oracle's {itn}th: {dev_k:.3e}",
+          flush=True)
+    if precision != "mixed":
+        assert abs(itn - itn_ref) <= 1, (itn, itn_ref)
+    if precision == "fp64":
+        assert np.abs(x - x_ref).max() <= 1e-9 * scale
+    assert dev_k <= bounds[precision], (precision, dev_k)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "refined"])
 def test_minres_matches_oracle(precision):
     from materialite_amd._native import NativeSolver
     from oracle.minres import minres
@@ -151,7 +173,7 @@ def test_minres_matches_oracle(precision):
         return apply_projection(np.einsum("pmn,pn->pm", tangent, x.reshape(512, 6)), (8, 8, 8),
                                 stress_mask=mask).ravel()
 
-    rtol = 1e-7 if precision == "fp64" else 1e-5
+    rtol = 1e-5 if precision == "mixed" else 1e-7
     x_ref, itn_ref, _ = minres(lhs, b.ravel(), rtol=rtol, maxiter=5 * 6 * 512)
     sol = NativeSolver((8, 8, 8), precision)
     sol.add_elastic(F.TI_STIFFNESS)
@@ -163,17 +185,14 @@ def test_minres_matches_oracle(precision):
     itn = sol.minres_dev(bd.data_ptr(), xd.data_ptr(), mask, rtol)
     x = xd.cpu().numpy().T.ravel()
     sol.close()
-    scale = np.abs(x_ref).max()
-    if precision == "fp64":
-        assert abs(itn - itn_ref) <= 1, (itn, itn_ref)
-        assert np.abs(x - x_ref).max() <= 1e-9 * scale
-    else:
-        # an FP32 Krylov solve to rtol 1e-5 against the FP64 one to 1e-5
-        assert itn <= 2 * itn_ref + 5, (itn, itn_ref)
-        assert np.abs(x - x_ref).max() <= 1e-3 * scale
+    # the pore makes the system singular: FP32 Lanczos vectors drift most here
+    # (measured 5.0e-4 after 45 iterations; refined 7.5e-6 after 85, where the
+    # FP32 w and x vectors dominate)
+    _check_minres(precision, x, itn, x_ref, itn_ref, lhs, b.ravel(),
+                  {"fp64": 1e-9, "refined": 3e-5, "mixed": 2e-3})
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "refined"])
 def test_minres_partial_point_block(precision):
     """5 x 6 x 7 = 210 points: three 64-point blocks and a partial one.  The
     Krylov vectors are stored blocked by 64 points; the padding of the last
@@ -198,7 +217,7 @@ def test_minres_partial_point_block(precision):
         return apply_projection(np.einsum("pmn,pn->pm", tangent, x.reshape(n, 6)), dims,
                                 stress_mask=mask).ravel()
 
-    rtol = 1e-7 if precision == "fp64" else 1e-5
+    rtol = 1e-5 if precision == "mixed" else 1e-7
     x_ref, itn_ref, _ = minres(lhs, b.ravel(), rtol=rtol, maxiter=5 * 6 * n)
     sol = NativeSolver(dims, precision)
     sol.add_elastic(F.TI_STIFFNESS)
@@ -209,16 +228,11 @@ def test_minres_partial_point_block(precision):
     itn = sol.minres_dev(bd.data_ptr(), xd.data_ptr(), mask, rtol)
     x = xd.cpu().numpy().T.ravel()
     sol.close()
-    scale = np.abs(x_ref).max()
-    if precision == "fp64":
-        assert abs(itn - itn_ref) <= 1, (itn, itn_ref)
-        assert np.abs(x - x_ref).max() <= 1e-9 * scale
-    else:
-        assert itn <= 2 * itn_ref + 5, (itn, itn_ref)
-        assert np.abs(x - x_ref).max() <= 1e-3 * scale
+    _check_minres(precision, x, itn, x_ref, itn_ref, lhs, b.ravel(),
+                  {"fp64": 1e-9, "refined": 1e-6, "mixed": 1e-4})
 
 
-@pytest.mark.parametrize("precision", ["fp64", "mixed", "mixed-fft64"])
+@pytest.mark.parametrize("precision", ["fp64", "mixed", "refined"])
 @pytest.mark.parametrize("rate_exponent", [10.0, 10.5])
 def test_evp_update_matches_oracle(precision, rate_exponent):
     """rate exponent 10: |tau/g|^9 by squaring; 10.5: the general pow()."""

@@ -61,7 +61,7 @@ def with_pore(material, model):
     return m.create_regional_fields("phase", table)
 
 
-PRECISIONS = ["fp64", "mixed", "mixed-fft64"]
+PRECISIONS = ["fp64", "mixed", "refined"]
 
 
 def rtol_for(precision):

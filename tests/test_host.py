@@ -32,7 +32,7 @@ def test_library_exports_every_declared_symbol():
 def test_step_struct_layout_matches_header():
     import ctypes
     assert ctypes.sizeof(_native.StepParams) == 4 * 6 * 8 + 5 * 8 + 2 * 4
-    assert ctypes.sizeof(_native.StepReport) == 3 * 4 + 4 + 8 + 4 + 4 + 4 * 8
+    assert ctypes.sizeof(_native.StepReport) == 3 * 4 + 4 + 8 + 4 + 4 + 5 * 8
 
 
 def test_material_fixture_matches_oracle_voronoi():
