@@ -142,13 +142,19 @@ def serving_stream(nreq, rate, slots):
     return summ
 
 
+# bytes per stored real value: tangent and MINRES w / x vectors, Lanczos
+# vectors, spectrum (refined: an FP32 value plus a bf16 remainder)
+WIDTHS = {"fp64": (8, 8, 8), "mixed": (4, 4, 4), "refined": (4, 6, 6)}
+
+
 def bytes_per_vi(nz, precision):
     """Algorithmic HBM bytes per voxel-iteration of the minimal fused pipeline:
-    tangent (21), Krylov input and output (6 + 6), four passes over the
-    6-component half spectrum, and eight Mandel vectors of MINRES recurrences."""
-    k = 8 if precision == "fp64" else 4
+    tangent (21), Krylov input and output (6 + 6 Lanczos values), four passes
+    over the 6-component half spectrum, and the MINRES recurrences (read r1,
+    r2, w1, w2, x; write r2', w, x: three Lanczos and five w / x vectors)."""
+    k, kl, ks = WIDTHS[precision]
     h = (nz // 2 + 1) / nz
-    return 21 * k + 12 * k + 4 * 6 * 2 * k * h + 8 * 6 * k
+    return 21 * k + 12 * kl + 4 * 6 * 2 * ks * h + 6 * (3 * kl + 5 * k)
 
 
 def _timed_run(material, schedule, model, precision, end_time, dt, **run_kw):
@@ -439,10 +445,17 @@ def print_trace_summary(trace_csv):
     from collections import defaultdict
 
     durs = defaultdict(list)
-    for r in csv.DictReader(open(trace_csv)):
-        name = r["Kernel_Name"].replace("mtl::(anonymous namespace)::", "").replace("mtl::", "")
+    if trace_csv.endswith(".db"):  # rocprofv3's default (rocpd, SQLite) output
+        import sqlite3
+
+        rows = sqlite3.connect(trace_csv).execute("select name, start, end from kernels")
+    else:
+        rows = ((r["Kernel_Name"], r["Start_Timestamp"], r["End_Timestamp"])
+                for r in csv.DictReader(open(trace_csv)))
+    for kname, start, end in rows:
+        name = kname.replace("mtl::(anonymous namespace)::", "").replace("mtl::", "")
         name = re.sub(r"\(.*", "", name.replace("void ", ""))
-        durs[name].append((int(r["End_Timestamp"]) - int(r["Start_Timestamp"])) / 1e3)
+        durs[name].append((int(end) - int(start)) / 1e3)
     total = sum(sum(v) for v in durs.values())
     print("| kernel | dispatches | total us | avg us | working dispatches | working avg us | % |")
     print("|---|---:|---:|---:|---:|---:|---:|")
@@ -501,14 +514,14 @@ def measured_traffic(kernel):
 
 
 def kernel_bytes(n, nzc_frac, precision):
-    """Algorithmic HBM bytes per launch of each pass kernel (FFT/Krylov type)."""
-    esz = 8 if precision == "fp64" else 4
-    spec = 6 * n * nzc_frac * 2 * esz  # one read or write of the 6-component half spectrum
+    """Algorithmic HBM bytes per launch of each pass kernel."""
+    k, kl, ks = WIDTHS[precision]
+    spec = 6 * n * nzc_frac * 2 * ks  # one read or write of the 6-component half spectrum
     return {
-        "z_forward": n * (6 + 21) * esz + spec,  # r2 (6) and tangent (21) read, spectrum written
+        "z_forward": n * (6 * kl + 21 * k) + spec,  # r2 and tangent read, spectrum written
         "y_pass": 2 * spec,
         "x_ghat": 2 * spec,
-        "z_inverse": spec + n * 3 * 6 * esz,  # spectrum, r1 and r2 read, u written
+        "z_inverse": spec + n * 3 * 6 * kl,  # spectrum, r1 and r2 read, u written
     }
 
 

@@ -263,6 +263,46 @@ __device__ __forceinline__ void butterfly(const cplx<T>* __restrict__ src, cplx<
   for (int u = 0; u < R; ++u) dst[(idx + u * Ns) * S + m] = v[u];
 }
 
+// A radix-R pass for any odd prime R without a dedicated butterfly: each
+// work item computes one output u of one butterfly (line m, index j) as the
+// direct sum over its R inputs,
+//   y_u = sum_t x_t w^(t kk tstride) w^(t u L/R)  (w = exp(-+2 pi i / L)),
+// with the combined twiddle exponent stepped modulo L, and writes it to the
+// autosorted position of the Stockham pass.  O(R) per output instead of
+// O(log R); used only for the prime factors above 13.
+template <typename T, int BT>
+__device__ __forceinline__ void generic_pass(const cplx<T>* __restrict__ src,
+                                             cplx<T>* __restrict__ dst, int M, int S, int L, int R,
+                                             int Ns, unsigned long long ns_magic,
+                                             const cplx<T>* tw, bool inv) {
+  const int per_line = L / R;
+  const int tstride = per_line / Ns;
+  for (int w = threadIdx.x; w < L * M; w += BT) {
+    const int m = w % M;
+    const int q = w / M;  // u * per_line + j
+    const int u = q / per_line, j = q - u * per_line;
+    const int jq = fast_div(j, ns_magic);
+    const int kk = j - jq * Ns;
+    int step = kk * tstride + u * per_line;
+    if (step >= L) step -= L;
+    cplx<T> acc = src[j * S + m];
+    int e = 0;
+    for (int t = 1; t < R; ++t) {
+      e += step;
+      if (e >= L) e -= L;
+      cplx<T> c = tw[e];
+      if (inv) c.y = -c.y;
+      acc = cadd(acc, cmul(src[(j + t * per_line) * S + m], c));
+    }
+    dst[(jq * Ns * R + kk + u * Ns) * S + m] = acc;
+  }
+}
+
+__host__ __device__ inline bool has_butterfly(int R) {
+  return R == 2 || R == 3 || R == 4 || R == 5 || R == 7 || R == 8 || R == 11 || R == 13 ||
+         R == 16;
+}
+
 // Transform M interleaved lines held in `a` (row stride S >= M complex values),
 // using `b` as the second buffer.  Returns the buffer that holds the result.
 // Ends with a barrier.  The radix switch sits inside the butterfly loop so that
@@ -287,6 +327,16 @@ __device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int
     const unsigned long long mg = pl->ns_magic[p];
     const int per_line = L / R;
     const int tstride = per_line / Ns;
+    if constexpr (BIG) {
+      if (!has_butterfly(R)) {
+        generic_pass<T, BT>(a, b, M, S, L, R, Ns, mg, tw, inv);
+        lds_barrier();
+        cplx<T>* t = a;
+        a = b;
+        b = t;
+        continue;
+      }
+    }
     int m = m0;
     for (int j = j0; j < per_line;) {
       const int jq = fast_div(j, mg);
@@ -322,8 +372,9 @@ __device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int
   return a;
 }
 
-// Host: factor L into radices 16,8,4,2,3,5,7,11,13 (largest powers of two first).
-// Throws if impossible.  uses_big_radix(): the plan needs the BIG block_fft.
+// Host: factor L into radices 16,8,4,2,3,5,7,11,13 (largest powers of two
+// first), then into the remaining primes, which run the generic pass.
+// uses_big_radix(): the plan needs the BIG block_fft (radix 11, 13 or generic).
 inline AxisPlan make_axis_plan(int L) {
   AxisPlan p{};
   p.L = L;
@@ -341,9 +392,18 @@ inline AxisPlan make_axis_plan(int L) {
       ++p.npass;
     }
   }
-  if (rem != 1)
-    throw Error("grid dimension " + std::to_string(L) +
-                " has a prime factor above 13; the FFT supports radices 2-5, 7, 8, 11, 13, 16");
+  for (int r = 17; rem > 1; r += 2) {
+    if (r > rem / r) r = rem;  // no factor up to sqrt(rem): rem is prime
+    while (rem % r == 0) {
+      if (p.npass >= kMaxPasses) throw Error("FFT plan: too many passes");
+      p.radix[p.npass] = r;
+      p.ns[p.npass] = ns;
+      p.ns_magic[p.npass] = div_magic(unsigned(ns));
+      ns *= r;
+      rem /= r;
+      ++p.npass;
+    }
+  }
   if (L >= (1 << 16)) throw Error("grid dimension " + std::to_string(L) + " too large");
   return p;
 }

@@ -164,6 +164,11 @@ class SmallStrainFFT:
             time_increment = self.initial_time_increment
             krylov = newton = 0
             floored = False
+            # every attempted increment: (time, dt, converged, Newton iterations,
+            # Krylov iterations, constitutive failure); kept up to date so that
+            # it survives a "min time increment reached" error
+            history = []
+            self.stats = {"history": history}
             while time < (self.end_time - time_tolerance):
                 self._logger.info(f"New increment: time {time} + {time_increment}")
                 d_strain = ls.strain_increment(time, time_increment).components
@@ -190,6 +195,9 @@ class SmallStrainFFT:
                         f"{linear_solver_tolerance:g} raised to {rep.linear_solver_tolerance_used:g}")
                 krylov += rep.krylov_iterations
                 newton += rep.newton_iterations
+                history.append((float(time), float(time_increment), bool(rep.converged),
+                                int(rep.newton_iterations), int(rep.krylov_iterations),
+                                bool(rep.constitutive_failed)))
                 self._logger.debug(f"global iterations {rep.newton_iterations}, "
                                    f"linear solver iterations {rep.krylov_iterations}, "
                                    f"equilibrium error {rep.equilibrium_error}")
@@ -237,7 +245,7 @@ class SmallStrainFFT:
             outputs.update({"stress": Order2SymmetricTensor(native.get_field("stress"), "p"),
                             "strain": Order2SymmetricTensor(native.get_field("strain"), "p")})
             self.stats = {"krylov_iterations": int(krylov), "newton_iterations": int(newton),
-                          "steps": int(time_step_id),
+                          "steps": int(time_step_id), "history": history,
                           "linear_solver_tolerance_used": max(
                               float(linear_solver_tolerance),
                               1.0e-5 if self.precision == "mixed" else 0.0)}

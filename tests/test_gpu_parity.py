@@ -31,7 +31,8 @@ def _need_gpu():
 GRIDS = [(3, 4, 5), (4, 4, 4), (2, 3, 6), (1, 4, 3), (6, 5, 8), (8, 8, 8), (7, 9, 10),
          (16, 16, 16), (12, 10, 14), (32, 32, 32), (48, 48, 48),
          (256, 4, 6), (256, 2, 40), (250, 3, 4), (512, 2, 6), (256, 3, 32),
-         (11, 13, 4), (22, 26, 12), (13, 11, 26)]  # radix-11/13 passes on every axis
+         (11, 13, 4), (22, 26, 12), (13, 11, 26),  # radix-11/13 passes on every axis
+         (17, 4, 6), (19, 19, 19), (2, 23, 34), (31, 6, 37)]  # generic prime passes
 
 
 @pytest.mark.parametrize("dims", GRIDS)
@@ -59,13 +60,27 @@ def test_projection_matches_oracle(dims, precision):
         assert np.abs(got - ref).max() <= tol * scale, (dims, precision, np.abs(got - ref).max() / scale)
 
 
-def test_unsupported_prime_factor_is_an_error():
-    """Grid lengths with a prime factor above 13 are rejected with a message
-    (no silent fallback)."""
-    from materialite_amd._native import NativeSolver
+def test_prime_grid_solve_matches_oracle():
+    """A 17^3 EVP solve (prime length on every axis: the generic FFT pass)
+    against the oracle's numpy.fft path."""
+    from materialite_amd import serving
+    from materialite_amd.models.small_strain_fft import LoadSchedule, SmallStrainFFT
+    from oracle.solver import LoadSchedule as OLS
+    from oracle.solver import SmallStrainFFT as OSolver
+    from tests.test_gpu_cutback import _oracle_evp
 
-    with pytest.raises(Exception, match="prime factor above 13"):
-        NativeSolver((17, 4, 4), "fp64")
+    print("17^3 EVP solve", flush=True)
+    material = serving.grain_material(17, 6, 20260017)
+    rot = material.extract("orientation").rotation_matrix.reshape(-1, 3, 3)
+    ref = OSolver(OLS.from_constant_uniaxial_strain_rate(1e-3, "x"), end_time=0.6,
+                  initial_time_increment=0.2).run([17] * 3, rot, _oracle_evp())
+    for precision, tol in (("fp64", 1e-9), ("refined", 1e-6)):
+        out = SmallStrainFFT(LoadSchedule.from_constant_uniaxial_strain_rate(1e-3, "x"),
+                             end_time=0.6, initial_time_increment=0.2,
+                             constitutive_model=serving.config3_evp(), precision=precision)(material)
+        err = np.abs(out.extract("stress").components - ref["stress"]).max() / np.abs(ref["stress"]).max()
+        print(f"  {precision}: {err:.3e}", flush=True)
+        assert err <= tol, (precision, err)
 
 
 def test_invalid_model_arguments_are_errors():
