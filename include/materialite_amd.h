@@ -188,6 +188,20 @@ int mtl_profile_read(int kernel, double* total_ms, long long* launches);
  * number recorded.  No reference counterpart (measurement only). */
 int mtl_profile_durations(int kernel, float* ms_out, long long capacity, long long* count);
 
+/* ---- projection without device allocation ----
+ * No counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Computes what mtl_solver_project computes for a handle of
+ * the same grid and precision, with the spectrum and FFT tables in a fixed
+ * device array of MTL_NOALLOC_SCRATCH_BYTES per GPU (held until the process
+ * ends) instead of a handle's buffers.  Grids whose spectrum and tables need
+ * more return an error (FP64: up to about 5.5e5 points, e.g. 80^3; the FP32
+ * forms about twice that).  Runs on the null stream of the current device and
+ * returns when done; calls that overlap on one GPU share the array and race. */
+#define MTL_NOALLOC_SCRATCH_BYTES (64u << 20)
+int mtl_project_noalloc(int nx, int ny, int nz, int precision, const double* in_dev,
+                        double* out_dev, const double stress_mask[6]);
+
 #ifdef __cplusplus
 }
 #endif

@@ -108,6 +108,8 @@ EXPORTS = {
     "mtl_solver_constitutive_host": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p,
                                                     ctypes.c_void_p, ctypes.c_double,
                                                     ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
+    "mtl_project_noalloc": (ctypes.c_int, [ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                           ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "mtl_profile_enable": (ctypes.c_int, [ctypes.c_int]),
     "mtl_profile_reset": (ctypes.c_int, []),
     "mtl_profile_read": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_double),

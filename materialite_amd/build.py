@@ -13,7 +13,7 @@ from concurrent.futures import ThreadPoolExecutor
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "libmaterialite_amd.so")
-SOURCES = ["spectral.hip", "minres.hip", "constitutive.hip", "solver.hip"]
+SOURCES = ["spectral.hip", "minres.hip", "constitutive.hip", "solver.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 FLAGS = [
     "-O3",
