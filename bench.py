@@ -18,18 +18,31 @@ launching stream during the timed region), against 8 TB/s.  For scale the entry
 also carries the rate of a plain device-to-device copy of 512 MiB measured in
 the same process (torch's copy kernel, read + write bytes over its duration).
 
-fp64_form: the plain FP64 form on the same increments, the mixed form's stress
-error against it (maximum, and per-point quantiles), the same figures for the
-FP64 form run with the load rate changed by 1 part in 1e7 (the floor that EVP
-convergence-test flips set on the maximum), and both forms' distance to an FP64
-solution of the same increments converged to Newton 1e-6 / MINRES 1e-8
-(converged_reference); also
-the "mixed-fft64" form (the mixed form's FP32 data with FP64 arithmetic inside
-the FFT passes) on the same increments (mixed_fft64_form).
+Precision forms: "refined" (the headline; FP32 tangent and MINRES w / x,
+the spectrum and Lanczos vectors stored as FP32 plus a bf16 remainder, FP64
+arithmetic in the FFT passes), "mixed" (FP32 storage and arithmetic), "fp64".
 
-cpu_baseline: the oracle (NumPy FP64 restatement of the reference path) timed
-on this host for a bounded number of MINRES iterations of the first Newton
-correction of the same 128^3 workload, single-threaded.
+roofline: the dominant kernel over all kernels of the timed increments; for a
+matvec pass or MINRES vector kernel its algorithmic bytes per launch over its
+mean working launch time; traffic: measured HBM bytes per launch (rocprofv3
+FETCH_SIZE + WRITE_SIZE, committed summary).  evp_iteration: the EVP iteration
+kernel's share, its measured HBM rate and FP64 VALU rate (rocprofv3
+SQ_INSTS_VALU_FLOPS_FP64 per working dispatch over its mean working time) and
+VALU busy / lane utilisation.
+
+fp64_form: the plain FP64 form on the same increments, the headline form's
+stress (and slip resistance) error against it (maximum, and per-point
+quantiles), the same figures for the FP64 form run with the load rate changed
+by 1 part in 1e7 (the floor that EVP convergence-test flips set on the
+maximum), both forms' distance to an FP64 solution of the same increments
+converged to Newton 1e-6 / MINRES 1e-8 (converged_reference), and the other
+FP32-storage form on the same increments.  full_path_50_increments: the whole
+load path (increments 2-50).
+
+cpu_baseline: the oracle (NumPy FP64 restatement of the reference path), one
+thread on this host: the first Newton iteration of the same 128^3 workload
+(EVP update of every point, residual projection, a bounded number of MINRES
+iterations).
 """
 
 import argparse
@@ -58,6 +71,8 @@ def parse():
     ap.add_argument("--compare-fp64", type=int, default=1,
                     help="also time the plain FP64 form on the same increments")
     ap.add_argument("--cpu-baseline", type=int, default=1)
+    ap.add_argument("--full-path", type=int, default=1,
+                    help="also time the whole 50-increment load path (increments 2-50)")
     ap.add_argument("--stream-requests", type=int, default=32,
                     help="requests of the mixed-size serving stream (0 = skip)")
     ap.add_argument("--stream-rate", type=float, default=12.0,
@@ -67,7 +82,7 @@ def parse():
     ap.add_argument("--precision-study", type=int, default=1,
                     help="64^3 elastic, uniaxial x then xy shear: mixed vs FP64 stress error")
     ap.add_argument("--large", type=int, default=1,
-                    help="also run the 256^3 two-phase EVP solve (mixed, mixed-fft64 and FP64; "
+                    help="also run the 256^3 two-phase EVP solve (refined, mixed and FP64; "
                          "about a minute)")
     ap.add_argument("--large-converged", type=int, default=0,
                     help="with --large: also solve the 256^3 case with FP64 at Newton 1e-6 / "
@@ -77,6 +92,8 @@ def parse():
                          "dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-dispatch HBM traffic summary of two rocprofv3 --pmc runs")
+    ap.add_argument("--summarize-valu", metavar="COUNTER_CSV",
+                    help="print the EVP iteration kernel's VALU summary of a rocprofv3 --pmc run")
     ap.add_argument("--stream-only", action="store_true",
                     help="run only the serving stream and print its summary (for tracing it)")
     ap.add_argument("--summarize-stream-trace", metavar="KERNEL_TRACE_CSV",
@@ -227,44 +244,34 @@ def precision_study(n=64, grains=200, steps=10):
 
 
 def large_two_phase(n=256, grains=1000, steps=30, dt=0.2, converged=False):
-    """256^3 two-phase EVP: 1000-grain FCC matrix (seed 20260005) with 10 %
-    volume fraction of periodic non-overlapping spheres of radius 8, elastic
-    isotropic with K, mu = 10 x the matrix Voigt averages; uniaxial z 1e-3/s,
-    30 steps of 0.2 s (the increment adapts as in the reference); mixed and
-    mixed-fft64 vs plain FP64 form."""
+    """256^3 two-phase EVP (serving.two_phase_material, seed 20260005):
+    1000-grain FCC matrix with 10 % volume fraction of periodic non-overlapping
+    spheres of radius 8, elastic isotropic with K, mu = 10 x the matrix Voigt
+    averages; uniaxial z 1e-3/s, 30 steps of 0.2 s (the increment adapts as in
+    the reference); refined and mixed vs the plain FP64 form."""
     import numpy as np
-    import pandas as pd
 
-    from materialite_amd import Material, Order4SymmetricTensor, Orientation, serving
-    from materialite_amd.microstructure import grain_rotations, periodic_spheres, periodic_voronoi
-    from materialite_amd.models.small_strain_fft import Elastic, LoadSchedule
+    from materialite_amd import serving
+    from materialite_amd.models.small_strain_fft import LoadSchedule
 
-    rng = np.random.default_rng(20260005)
-    grain_of_point, _ = periodic_voronoi([n] * 3, grains, rng)
-    rot = grain_rotations(grains, rng)
-    inclusion, centres = periodic_spheres([n] * 3, 8.0, 0.10, rng)
-    c11, c12, c44 = 250e3, 150e3, 120e3
-    k_v = (c11 + 2 * c12) / 3
-    mu_v = (c11 - c12 + 3 * c44) / 5
-    k, mu = 10 * k_v, 10 * mu_v
-    e = 9 * k * mu / (3 * k + mu)
-    m = Material(dimensions=[n] * 3).create_fields(
-        {"grain": grain_of_point, "phase": inclusion.astype(np.int64)})
-    m = m.create_regional_fields("grain", {"grain": np.arange(grains),
-                                           "orientation": Orientation(rot, "p")})
-    table = pd.DataFrame({"phase": [0, 1], "constitutive_model": [
-        serving.config3_evp(), Elastic(Order4SymmetricTensor.from_isotropic_constants(e, mu))]})
-    m = m.create_regional_fields("phase", table)
+    m, spheres = serving.two_phase_material(n, grains, 20260005)
+    vf = float(np.mean(m.extract("phase") == 1))
     ls = LoadSchedule.from_constant_uniaxial_strain_rate(1e-3, "z")
-    res = {}
-    incs = {}
+    res, incs = {}, {}
     for prec in ("refined", "mixed", "fp64"):
         print(f"[bench] 256^3 two-phase: {prec}", file=sys.stderr, flush=True)
         res[prec] = _timed_run(m, ls, None, prec, steps * dt, dt, phase_label="phase")
         incs[prec] = max(_timed_run.increments, 1)
-    (sm, km, tm), (sd, kd, td) = res["mixed"], res["fp64"]
-    sa, ka, ta = res["refined"]
-    vim, vid = n ** 3 * km / tm, n ** 3 * kd / td
+    sd, kd, td = res["fp64"]
+
+    def form(prec):
+        s_, k_, t_ = res[prec]
+        return {"max_rel_stress_error": float(np.abs(s_ - sd).max() / np.abs(sd).max()),
+                "stress_error_profile": stress_error_profile(s_, sd),
+                "krylov": int(k_), "vi_per_s": n ** 3 * k_ / t_,
+                "ms_per_increment": 1e3 * t_ / incs[prec],
+                "roofline_frac": n ** 3 * k_ / t_ * bytes_per_vi(n, prec) / (HBM_PEAK_GBS * 1e9)}
+
     conv = None
     if converged:
         print("[bench] 256^3 two-phase: fp64, Newton 1e-6 / MINRES 1e-8", file=sys.stderr,
@@ -274,26 +281,16 @@ def large_two_phase(n=256, grains=1000, steps=30, dt=0.2, converged=False):
         scale = np.abs(st).max()
         conv = dict(tight_kw, krylov_iterations=int(kt),
                     max_rel_stress_error_of_fp64=float(np.abs(sd - st).max() / scale),
-                    max_rel_stress_error_of_mixed=float(np.abs(sm - st).max() / scale))
+                    max_rel_stress_error_of_refined=float(np.abs(res["refined"][0] - st).max() / scale))
     return {
-        "workload": f"{n}^3 two-phase EVP, {grains} grains, {len(centres)} spheres r 8 "
-                    f"({inclusion.mean():.3f} vf), uniaxial z, {steps * dt:g} s from an "
+        "workload": f"{n}^3 two-phase EVP, {grains} grains, {spheres} spheres r 8 "
+                    f"({vf:.3f} vf), uniaxial z, {steps * dt:g} s from an "
                     f"increment of {dt} s (at most {dt} s; it adapts)",
-        "max_rel_stress_error": float(np.abs(sm - sd).max() / np.abs(sd).max()),
-        "stress_error_profile": stress_error_profile(sm, sd),
-        "krylov_mixed": int(km), "krylov_fp64": int(kd),
-        "vi_per_s_mixed": vim, "vi_per_s_fp64": vid,
-        "effective_vi_per_s_mixed": n ** 3 * kd / tm,
-        "timed_increments": incs["mixed"],
-        "ms_per_increment_mixed": 1e3 * tm / incs["mixed"],
+        "timed_increments": incs["refined"],
+        "refined_form": form("refined"),
+        "mixed_form": form("mixed"),
+        "krylov_fp64": int(kd), "vi_per_s_fp64": n ** 3 * kd / td,
         "ms_per_increment_fp64": 1e3 * td / incs["fp64"],
-        "roofline_frac_mixed": vim * bytes_per_vi(n, "mixed") / (HBM_PEAK_GBS * 1e9),
-        "mixed_fft64_form": {
-            "max_rel_stress_error": float(np.abs(sa - sd).max() / np.abs(sd).max()),
-            "stress_error_profile": stress_error_profile(sa, sd),
-            "krylov": int(ka), "vi_per_s": n ** 3 * ka / ta,
-            "ms_per_increment": 1e3 * ta / incs["refined"],
-        },
         "converged_reference": conv,
     }
 
@@ -375,8 +372,9 @@ def time_increments(material, precision, warmup, steps, profile, rate=1e-3, **ru
 
     if warmup == 0:
         raise SystemExit("--warmup must be >= 1 (the first increment builds the solver state)")
-    out = model.run(material, increment_callback=cb, **run_kw)
+    out = model.run(material, increment_callback=cb, output_variables=["slip_resistances"], **run_kw)
     rec["stress"] = out.extract("stress").components
+    rec["slip_resistances"] = out.extract("slip_resistances").components
     if profile:
         rec["profile"] = _native.profile_read()
         rec["durations"] = {k: _native.profile_durations(k) for k in _native.KERNELS}
@@ -385,10 +383,14 @@ def time_increments(material, precision, warmup, steps, profile, rate=1e-3, **ru
     return rec
 
 
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_evp128_mixed.txt")
-KERNEL_SYMBOLS = {"z_forward": "k_z_forward<float, float, 1, float", "y_pass": "k_y_pass<float, float",
-                  "x_ghat": r"k_x_ghat_\w+<float, float", "z_inverse": "k_z_inverse<float, float, 1, float",
-                  "evp_iteration": "k_evp_iter<12, true"}
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "pmc_traffic_evp128_refined.txt")
+VALU_FILE = os.path.join(ROOT, "profiles", "pmc_valu_evp128_refined.txt")
+# kernel-name patterns of the refined form's matvec passes and EVP iteration
+KERNEL_SYMBOLS = {"z_forward": r"k_z_forward<double, mtl::f48, 1,", "y_pass": r"k_y_pass<double, mtl::f48",
+                  "x_ghat": r"k_x_ghat_\w+<double, mtl::f48", "z_inverse": r"k_z_inverse<double, mtl::f48, 1,",
+                  "minres_vector": r"k_(lanczos|update)<mtl::f48", "evp_iteration": r"k_evp_iter<12, true"}
+FP64_PEAK_TFLOPS = 78.6
+NUM_CUS = 256
 
 
 def summarize_pmc(fetch_csv, write_csv):
@@ -500,17 +502,60 @@ def print_stream_trace_summary(trace_csv):
     print(f"most kernels in flight together: {peak}")
 
 
+def _table_row(path, kernel, width):
+    try:
+        for line in open(path):
+            cells = [c.strip() for c in line.strip().strip("|").split("|")]
+            if len(cells) == width and cells[0] == kernel:
+                return cells
+    except OSError:
+        return None
+    return None
+
+
 def measured_traffic(kernel):
     """HBM bytes per launch of `kernel` from the committed rocprofv3 summary
     (scaled FETCH + WRITE), or None."""
-    try:
-        for line in open(TRAFFIC_FILE):
-            cells = [c.strip() for c in line.strip().strip("|").split("|")]
-            if len(cells) == 4 and cells[0] == kernel:
-                return float(cells[2]) + float(cells[3])
-    except (OSError, ValueError):
-        return None
-    return None
+    cells = _table_row(TRAFFIC_FILE, kernel, 4)
+    return None if cells is None else float(cells[2]) + float(cells[3])
+
+
+def measured_valu(kernel):
+    """(FP64 FLOPs per working dispatch, VALU busy %, VALU lane utilisation %)
+    of `kernel` from the committed rocprofv3 counter summary, or None."""
+    cells = _table_row(VALU_FILE, kernel, 5)
+    return None if cells is None else (float(cells[2]), float(cells[3]), float(cells[4]))
+
+
+def print_valu_summary(csv_path):
+    """Per working dispatch of the EVP iteration kernel: FP64 FLOPs
+    (SQ_INSTS_VALU_FLOPS_FP64), VALU busy (SQ_ACTIVE_INST_VALU over
+    GRBM_GUI_ACTIVE and the CU count) and VALU lane utilisation
+    (SQ_THREAD_CYCLES_VALU over 64 x SQ_ACTIVE_INST_VALU)."""
+    import csv
+    import re
+    from collections import defaultdict
+
+    per = defaultdict(lambda: defaultdict(float))
+    names = {}
+    for r in csv.DictReader(open(csv_path)):
+        key = r.get("Dispatch_Id") or r.get("Correlation_Id")
+        per[key][r["Counter_Name"]] += float(r["Counter_Value"])
+        names[key] = r["Kernel_Name"]
+    print("| kernel | working dispatches | FP64 FLOPs / dispatch | VALU busy % | VALU lane utilisation % |")
+    print("|---|---:|---:|---:|---:|")
+    for name in ("evp_iteration",):
+        rows = [v for k, v in per.items() if re.search(KERNEL_SYMBOLS[name], names[k])]
+        if not rows:
+            continue
+        top = max(r.get("SQ_INSTS_VALU_FLOPS_FP64", 0.0) for r in rows)
+        rows = [r for r in rows if r.get("SQ_INSTS_VALU_FLOPS_FP64", 0.0) > 0.2 * top]
+        flops = sum(r["SQ_INSTS_VALU_FLOPS_FP64"] for r in rows) / len(rows)
+        busy = sum(100.0 * r["SQ_ACTIVE_INST_VALU"] / NUM_CUS / max(r["GRBM_GUI_ACTIVE"], 1.0)
+                   for r in rows) / len(rows)
+        util = sum(100.0 * r["SQ_THREAD_CYCLES_VALU"] / (64.0 * max(r["SQ_ACTIVE_INST_VALU"], 1.0))
+                   for r in rows) / len(rows)
+        print(f"| {name} | {len(rows)} | {flops:.4g} | {busy:.1f} | {util:.1f} |")
 
 
 def kernel_bytes(n, nzc_frac, precision):
@@ -522,37 +567,50 @@ def kernel_bytes(n, nzc_frac, precision):
         "y_pass": 2 * spec,
         "x_ghat": 2 * spec,
         "z_inverse": spec + n * 3 * 6 * kl,  # spectrum, r1 and r2 read, u written
+        # k_lanczos (u read and written, r2 read) and k_update (v read; w1, w2,
+        # x read; w, x written) alternate: their mean per launch
+        "minres_vector": n * 6 * (3 * kl + kl + 5 * k) / 2,
     }
 
 
 def cpu_baseline_child(n, grains, iters):
+    """The oracle (NumPy FP64, one thread) on the first Newton iteration of the
+    128^3 EVP workload's first increment: untimed set-up (orientations,
+    initialize), then timed: the EVP update of every point from the predictor
+    (all its implicit iterations), the residual projection and `iters` MINRES
+    iterations of the Newton correction."""
     os.environ.setdefault("OMP_NUM_THREADS", "1")
     import numpy as np
 
+    from oracle import constitutive as C
     from oracle import tensors as T
     from oracle.minres import minres
     from oracle.projection import apply_projection
 
     grain_of_point, rot = workload(n, grains)
     npts = n ** 3
-    c = T.cubic_stiffness(250e3, 150e3, 120e3)
-    rm = T.rotation_matrix_mandel(rot)
-    tangent_g = T.order4_to_specimen(np.broadcast_to(c, rm.shape), rm)
-    tangent = tangent_g[grain_of_point]
+    model = C.ElasticViscoplastic(T.cubic_stiffness(250e3, 150e3, 120e3),
+                                  T.schmid_mandel(*T.octahedral_slip_systems()), 1e-3, 20.0, 50.0,
+                                  "voce", {"tau_1": 100.0, "theta_0": 500.0, "theta_1": 0.0})
+    tangent = model.initialize(rot[grain_of_point])
     mask = np.array([0, 1, 1, 1, 1, 1.0])
-    d_strain = np.array([2e-4, 0, 0, 0, 0, 0])
-    sigma = np.einsum("pmn,n->pm", tangent, d_strain)
-    b = apply_projection(-sigma, (n, n, n), stress_mask=mask).ravel()
+    d_strain = np.array([2e-4, 0, 0, 0, 0, 0])  # 1e-3/s x 0.2 s
+    strain = np.broadcast_to(d_strain, (npts, 6)).copy()
+    guess = np.einsum("pmn,n->pm", tangent, d_strain)
+    t0 = time.perf_counter()
+    stress, tangent, evp_its = model.calculate_stress_and_tangent(strain, guess, 0.2)
+    t1 = time.perf_counter()
+    b = apply_projection(-stress, (n, n, n), stress_mask=mask).ravel()
 
     def lhs(x):
         return apply_projection(np.einsum("pmn,pn->pm", tangent, x.reshape(npts, 6)), (n, n, n),
                                 stress_mask=mask).ravel()
 
-    lhs(b)  # warm caches / page in
-    t0 = time.perf_counter()
     _, itn, _ = minres(lhs, b, rtol=1e-30, maxiter=iters)
-    dt = time.perf_counter() - t0
-    print(json.dumps({"vi_per_s": npts * itn / dt, "iterations": itn, "seconds": dt}))
+    t2 = time.perf_counter()
+    print(json.dumps({"vi_per_s": npts * itn / (t2 - t0), "iterations": itn,
+                      "evp_iterations": int(evp_its), "seconds": t2 - t0, "evp_seconds": t1 - t0,
+                      "krylov_seconds": t2 - t1}))
 
 
 def host_info():
@@ -670,8 +728,10 @@ def cpu_baseline(n, grains, iters):
         "cores": 1,
         "host": host_info(),
         "kind": "port",
-        "sample": (f"oracle FP64 NumPy MINRES, {res['iterations']} Krylov iterations of the first "
-                   f"Newton correction of the {n}^3 EVP workload ({res['seconds']:.1f} s, 1 thread)"),
+        "sample": (f"oracle FP64 NumPy, the first Newton iteration of the {n}^3 EVP workload: EVP "
+                   f"update of all points ({res['evp_iterations']} implicit iterations, "
+                   f"{res['evp_seconds']:.1f} s), residual projection and {res['iterations']} MINRES "
+                   f"iterations ({res['krylov_seconds']:.1f} s); {res['seconds']:.1f} s, 1 thread"),
     }
 
 
@@ -708,6 +768,9 @@ def main():
     if args.summarize_trace:
         print_trace_summary(args.summarize_trace)
         return
+    if args.summarize_valu:
+        print_valu_summary(args.summarize_valu)
+        return
     if args.summarize_pmc:
         print_pmc_summary(args.summarize_pmc[0], args.summarize_pmc[1], args.copy_probe or 512)
         return
@@ -741,14 +804,56 @@ def main():
     prof = prof_rec["profile"]
     nzc_frac = ((n + 1) // 2) / n
     kb = kernel_bytes(npts, nzc_frac, args.precision)
-    best = max(kb, key=lambda k: prof[k][0])
-    # mean duration of the launches that did work: launches enqueued past MINRES
-    # convergence return within a few microseconds (below 20 % of the longest)
-    durs = prof_rec["durations"][best]
-    working = durs[durs > 0.2 * durs.max()]
-    mean_s = float(working.mean()) / 1e3
-    achieved = kb[best] / mean_s / 1e9
+
+    def working_mean_s(kernel):
+        # mean duration of the launches that did work: launches enqueued past
+        # convergence return within a few microseconds (below 20 % of the longest)
+        durs = prof_rec["durations"][kernel]
+        if durs.size == 0:
+            return None, 0
+        working = durs[durs > 0.2 * durs.max()]
+        return float(working.mean()) / 1e3, int(working.size)
+
+    # the dominant kernel over all kernels of the timed increments
+    best = max(prof, key=lambda k: prof[k][0])
+    mean_s, launches = working_mean_s(best)
     copy_gbs = copy_ceiling()
+    if best in kb:
+        traffic = measured_traffic(best) if args.precision == "refined" and n == 128 else None
+        achieved = kb[best] / mean_s / 1e9
+        roofline = {"bound": "hbm", "kernel": best, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                    "algorithmic_bytes_per_launch": kb[best]}
+    else:
+        # the EVP iteration: its bytes per launch depend on the points still
+        # iterating, so the achieved rate uses the measured traffic per working
+        # dispatch (rocprofv3 FETCH_SIZE + WRITE_SIZE of the same workload)
+        traffic = measured_traffic(best) if args.precision == "refined" and n == 128 else None
+        achieved = traffic / mean_s / 1e9 if traffic else None
+        roofline = {"bound": "hbm", "kernel": best, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS if achieved else None,
+                    "traffic": traffic}
+    roofline.update({"mean_launch_us": mean_s * 1e6, "timed_launches": launches,
+                     "copy_gbs": copy_gbs,
+                     "frac_of_copy": roofline["achieved"] / copy_gbs if roofline["achieved"] else None})
+    # the EVP iteration kernel beside it: HBM rate and FP64 VALU rate per
+    # working dispatch (counters from the committed rocprofv3 summaries)
+    evp_s, evp_launches = working_mean_s("evp_iteration")
+    evp = None
+    if evp_s:
+        evp = {"ms": prof["evp_iteration"][0], "share_of_kernel_time": prof["evp_iteration"][0] /
+               sum(v[0] for v in prof.values()), "working_launches": evp_launches,
+               "mean_working_launch_us": evp_s * 1e6}
+        t_evp = measured_traffic("evp_iteration") if args.precision == "refined" and n == 128 else None
+        v_evp = measured_valu("evp_iteration") if args.precision == "refined" and n == 128 else None
+        if t_evp:
+            evp.update({"hbm_gbs": t_evp / evp_s / 1e9, "hbm_frac": t_evp / evp_s / 1e9 / HBM_PEAK_GBS,
+                        "traffic": t_evp})
+        if v_evp:
+            evp.update({"fp64_tflops": v_evp[0] / evp_s / 1e12,
+                        "fp64_frac": v_evp[0] / evp_s / 1e12 / FP64_PEAK_TFLOPS,
+                        "fp64_flops_per_dispatch": v_evp[0], "valu_busy_pct": v_evp[1],
+                        "valu_lane_utilisation_pct": v_evp[2]})
     fp64 = None
     if args.compare_fp64 and args.precision != "fp64":
         rec = time_increments(material, "fp64", args.warmup, args.steps, profile=False)
@@ -757,22 +862,24 @@ def main():
         # later (convergence tolerance 2e-5) move by up to ~2e-5 of the stress
         pert = time_increments(material, "fp64", args.warmup, args.steps, profile=False,
                                rate=1e-3 * (1 + 1e-7))
+        g_ref = rec["slip_resistances"]
         fp64 = {
-            "max_rel_stress_error_of_mixed": float(
+            "max_rel_stress_error_of_headline": float(
                 abs(main_rec["stress"] - rec["stress"]).max() / abs(rec["stress"]).max()),
-            "stress_error_profile_of_mixed": stress_error_profile(main_rec["stress"], rec["stress"]),
+            "stress_error_profile_of_headline": stress_error_profile(main_rec["stress"], rec["stress"]),
+            "max_rel_slip_resistance_error_of_headline": float(
+                abs(main_rec["slip_resistances"] - g_ref).max() / abs(g_ref).max()),
             "fp64_with_load_rate_x_1_plus_1e-7": stress_error_profile(pert["stress"], rec["stress"]),
             "value": npts * rec["krylov"] / rec["seconds"],
-            # the mixed form's rate counted with this form's iteration count
-            "effective_value_mixed": npts * rec["krylov"] / main_rec["seconds"],
+            # the headline form's rate counted with this form's iteration count
+            "effective_value_headline": npts * rec["krylov"] / main_rec["seconds"],
             "krylov_iterations": rec["krylov"],
             "seconds": rec["seconds"],
             "ms_per_step": 1e3 * rec["seconds"] / max(rec["steps"], 1),
         }
         # both forms against a tightly converged FP64 solution of the same
         # increments: at the default tolerances (Newton 1e-3, MINRES 1e-5) each
-        # form stops at its own iterate, and this shows how far each is from
-        # the converged state
+        # form stops at its own iterate
         tight_kw = {"global_tolerance": 1e-6, "linear_solver_tolerance": 1e-8}
         tight = time_increments(material, "fp64", args.warmup, args.steps, profile=False,
                                 **tight_kw)
@@ -781,20 +888,27 @@ def main():
             tight_kw,
             krylov_iterations=tight["krylov"],
             max_rel_stress_error_of_fp64=float(abs(rec["stress"] - tight["stress"]).max() / scale),
-            max_rel_stress_error_of_mixed=float(
+            max_rel_stress_error_of_headline=float(
                 abs(main_rec["stress"] - tight["stress"]).max() / scale))
-        if args.precision == "mixed":
-            f64a = time_increments(material, "refined", args.warmup, args.steps, profile=False)
-            fp64["mixed_fft64_form"] = {
-                "value": npts * f64a["krylov"] / f64a["seconds"],
-                "ms_per_step": 1e3 * f64a["seconds"] / max(f64a["steps"], 1),
-                "krylov_iterations": f64a["krylov"],
-                "max_rel_stress_error": float(
-                    abs(f64a["stress"] - rec["stress"]).max() / abs(rec["stress"]).max()),
-                "stress_error_profile": stress_error_profile(f64a["stress"], rec["stress"]),
-                "max_rel_stress_error_vs_converged": float(
-                    abs(f64a["stress"] - tight["stress"]).max() / scale),
-            }
+        other = "mixed" if args.precision == "refined" else "refined"
+        alt = time_increments(material, other, args.warmup, args.steps, profile=False)
+        fp64[f"{other}_form"] = {
+            "value": npts * alt["krylov"] / alt["seconds"],
+            "ms_per_step": 1e3 * alt["seconds"] / max(alt["steps"], 1),
+            "krylov_iterations": alt["krylov"],
+            "max_rel_stress_error": float(
+                abs(alt["stress"] - rec["stress"]).max() / abs(rec["stress"]).max()),
+            "stress_error_profile": stress_error_profile(alt["stress"], rec["stress"]),
+            "max_rel_slip_resistance_error": float(
+                abs(alt["slip_resistances"] - g_ref).max() / abs(g_ref).max()),
+        }
+    full = None
+    if args.full_path:
+        # the whole 50-increment load path (increment 1 builds the state)
+        fr = time_increments(material, args.precision, 1, 49, profile=False)
+        full = {"increments": fr["steps"], "krylov_iterations": fr["krylov"],
+                "value": npts * fr["krylov"] / fr["seconds"],
+                "ms_per_step": 1e3 * fr["seconds"] / max(fr["steps"], 1)}
     stream = (serving_stream(args.stream_requests, args.stream_rate, args.stream_slots)
               if args.stream_requests > 0 else None)
     study = precision_study() if args.precision_study else None
@@ -812,7 +926,10 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if args.precision == "fp64" else "f32",
+        "dtype": {"fp64": "f64", "mixed": "f32", "refined": "f64"}[args.precision],
+        "storage": {"fp64": "f64", "mixed": "f32",
+                    "refined": "f32 + bf16 remainder (spectrum, Lanczos vectors); f32 (tangent, "
+                               "MINRES w and x)"}[args.precision],
         "data": "synthetic",
         "config": {
             "workload": f"{n}^3 EVP FCC, {args.grains} periodic Voronoi grains, Voce, uniaxial x "
@@ -822,26 +939,17 @@ def main():
             "newton_iterations": main_rec["newton"],
             "seed": SEED,
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": best,
-            "achieved": achieved,
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": achieved / HBM_PEAK_GBS,
-            "traffic": measured_traffic(best) if args.precision == "mixed" and n == 128 else None,
-            "mean_launch_us": mean_s * 1e6,
-            "timed_launches": int(working.size),
-            "copy_gbs": copy_gbs,
-            "frac_of_copy": achieved / copy_gbs,
-        },
+        "roofline": roofline,
+        "evp_iteration": evp,
         "kernels_ms": {k: round(v[0], 3) for k, v in prof.items()},
         "kernels_launches": {k: int(v[1]) for k, v in prof.items()},
         "fp64_form": fp64,
+        "full_path_50_increments": full,
         "serving": stream,
         "precision_study_64": study,
         "two_phase_256": large,
         "vi_roofline_frac": vi * bytes_per_vi(n, args.precision) / (HBM_PEAK_GBS * 1e9),
+        "bytes_per_vi": bytes_per_vi(n, args.precision),
         "cpu_baseline": cpu,
         "config1_16": c1,
     }

@@ -3,7 +3,8 @@
 Restates ``SmallStrainFFT.run`` / ``_get_new_time_increment`` /
 ``_get_fluctuation_strain`` / ``_left_hand_side``
 (``materialite/models/small_strain_fft/small_strain_fft.py:60-299, 370-374``) and
-``LoadSchedule`` (``small_strain_fft/load_schedule.py:17-119``) on plain arrays:
+``LoadSchedule`` (``small_strain_fft/load_schedule.py:17-119``, all five
+factories) on plain arrays:
 fields are (N, 6) Mandel arrays in the reference's point order (x slowest, z
 fastest), tangents are (N, 6, 6).
 """
@@ -50,6 +51,47 @@ class LoadSchedule:
         mask = np.concatenate([masks[direction], np.ones(3)])
         rate = np.concatenate([rates[direction], np.zeros(3)])
         return cls.from_constant_rates(rate, np.zeros(6), mask, start_time)
+
+
+def _ramp(times, strains, stresses, stress_mask):
+    """load_schedule.py:63-75: linear in time through two (time, strain,
+    stress) states, each masked."""
+    mask = np.asarray(stress_mask, float)
+    e = np.asarray(strains, float) * (1 - mask)
+    s = np.asarray(stresses, float) * mask
+    t0, span = times[0], times[1] - times[0]
+    de, ds = (e[1] - e[0]) / span, (s[1] - s[0]) / span
+    return LoadSchedule(lambda t, dt: de * dt, lambda t, dt: ds * dt,
+                        lambda t, dt: ds * (t + dt - t0) + s[0], mask)
+
+
+def _cyclic(amplitude, frequency, start_time, ramp_rate):
+    a = np.asarray(amplitude, float)
+    return lambda t: a * (1 + ramp_rate * (t - start_time)) * np.sin(
+        2 * np.pi * frequency * (t - start_time))
+
+
+def _cyclic_stress(stress_amplitude, frequency, stress_mask=np.ones(6), start_time=0, ramp_rate=0):
+    """load_schedule.py:77-98: A(t) = a (1 + r (t - t0)) sin(2 pi f (t - t0)) on
+    the stress-controlled components; no strain increments."""
+    mask = np.asarray(stress_mask, float)
+    amp = _cyclic(np.asarray(stress_amplitude, float) * mask, frequency, start_time, ramp_rate)
+    return LoadSchedule(lambda t, dt: np.zeros(6), lambda t, dt: amp(t + dt) - amp(t),
+                        lambda t, dt: amp(t + dt), mask)
+
+
+def _cyclic_strain(strain_amplitude, frequency, stress_mask, start_time=0, ramp_rate=0):
+    """load_schedule.py:100-119: the same amplitude law on the strain-controlled
+    components; zero applied stress."""
+    mask = np.asarray(stress_mask, float)
+    amp = _cyclic(np.asarray(strain_amplitude, float) * (1 - mask), frequency, start_time, ramp_rate)
+    return LoadSchedule(lambda t, dt: amp(t + dt) - amp(t), lambda t, dt: np.zeros(6),
+                        lambda t, dt: np.zeros(6), mask)
+
+
+LoadSchedule.from_ramp = staticmethod(_ramp)
+LoadSchedule.from_cyclic_stress = staticmethod(_cyclic_stress)
+LoadSchedule.from_cyclic_strain = staticmethod(_cyclic_strain)
 
 
 def _von_mises_of_mean(stress):

@@ -68,7 +68,9 @@ def test_constitutive_failure_cutbacks_match_oracle(precision):
     assert len(hist) == len(ref_hist)
     for a, b in zip(hist, ref_hist):
         assert a[0] == pytest.approx(b[0], abs=1e-12) and a[1] == pytest.approx(b[1], abs=1e-12)
-        assert a[2:] == tuple(b[2:]), (a, b)
+        assert (a[2], a[3], a[5]) == (b[2], b[3], b[5]), (a, b)
+        # the refined form's MINRES stopping test may fall one iteration either side
+        assert abs(a[4] - b[4]) <= (0 if precision == "fp64" else 1), (a, b)
     stress = out.extract("stress").components
     tol = 1e-9 if precision == "fp64" else 1e-6
     assert np.abs(stress - ref["stress"]).max() <= tol * np.abs(ref["stress"]).max()
