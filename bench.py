@@ -153,7 +153,7 @@ def serving_stream(nreq, rate, slots):
         "voxel_iters_per_s": vi / span,
         "mix": {"evp": sum(sp.model == "evp" for sp in specs),
                 "grids": {str(n): sum(sp.n == n for sp in specs) for n in (32, 48, 64, 96)}},
-        "precision": "mixed",
+        "precision": "refined",
         "seeds": "request j: 20264000 + j; arrivals: 20264999",
     })
     return summ
@@ -205,8 +205,8 @@ def _timed_run(material, schedule, model, precision, end_time, dt, **run_kw):
 def precision_study(n=64, grains=200, steps=10):
     """64^3 elastic, 200 periodic-Voronoi grains (seed 20260002): uniaxial x,
     then pure xy shear, 10 steps of dt = 1 each, tolerances 1e-8 (Newton) and
-    1e-10 (MINRES); mixed vs plain FP64 form.  Error = max |sigma_mixed -
-    sigma_fp64| / max |sigma_fp64| over the whole field."""
+    1e-10 (MINRES); refined and mixed vs the plain FP64 form.  Error = max
+    |sigma_form - sigma_fp64| / max |sigma_fp64| over the whole field."""
     import numpy as np
 
     from materialite_amd import serving
@@ -219,12 +219,12 @@ def precision_study(n=64, grains=200, steps=10):
                        "tolerances 1e-8 / 1e-10"}
     for name, ls in loads.items():
         res = {}
-        for prec in ("mixed", "fp64"):
+        for prec in ("refined", "mixed", "fp64"):
             print(f"[bench] precision study: {name} {prec}", file=sys.stderr, flush=True)
             res[prec] = _timed_run(material, ls, serving.cubic_elastic(), prec, float(steps), 1.0,
                                    global_tolerance=1e-8, linear_solver_tolerance=1e-10)
-        (sm, km, tm), (sd, kd, td) = res["mixed"], res["fp64"]
-        vim, vid = n ** 3 * km / tm, n ** 3 * kd / td
+        sd, kd, td = res["fp64"]
+        vid = n ** 3 * kd / td
         print(f"[bench] precision study: {name} oracle, first step", file=sys.stderr, flush=True)
         cpu, _ = run_oracle({"n": n, "grains": grains, "seed": 20260002,
                              "load": "x" if name == "uniaxial_x" else "xy", "end_time": float(steps),
@@ -232,14 +232,15 @@ def precision_study(n=64, grains=200, steps=10):
         out[name + "_cpu_oracle_first_step"] = {
             "seconds": cpu["seconds"], "krylov": cpu["krylov"], "threads": 1,
             "vi_per_s": n ** 3 * cpu["krylov"] / cpu["seconds"]}
-        out[name] = {
-            "max_rel_stress_error": float(np.abs(sm - sd).max() / np.abs(sd).max()),
-            "krylov_mixed": int(km), "krylov_fp64": int(kd),
-            "vi_per_s_mixed": vim, "vi_per_s_fp64": vid,
-            "effective_vi_per_s_mixed": n ** 3 * kd / tm,
-            "gb_per_s_mixed": vim * bytes_per_vi(n, "mixed") / 1e9,
-            "gb_per_s_fp64": vid * bytes_per_vi(n, "fp64") / 1e9,
-        }
+        out[name] = {"krylov_fp64": int(kd), "vi_per_s_fp64": vid,
+                     "gb_per_s_fp64": vid * bytes_per_vi(n, "fp64") / 1e9}
+        for prec in ("refined", "mixed"):
+            sm, km, tm = res[prec]
+            vim = n ** 3 * km / tm
+            out[name][prec] = {
+                "max_rel_stress_error": float(np.abs(sm - sd).max() / np.abs(sd).max()),
+                "krylov": int(km), "vi_per_s": vim, "effective_vi_per_s": n ** 3 * kd / tm,
+                "gb_per_s": vim * bytes_per_vi(n, prec) / 1e9}
     return out
 
 
@@ -699,7 +700,7 @@ def config1():
                        "of dt 1, tolerances 1e-8 / 1e-10, whole run",
            "cpu_oracle_seconds": cpu["seconds"], "cpu_oracle_krylov": cpu["krylov"],
            "cpu_threads": 1}
-    for prec in ("fp64", "mixed"):
+    for prec in ("fp64", "refined", "mixed"):
         solver = SmallStrainFFT(LoadSchedule.from_constant_uniaxial_strain_rate(1e-3, "x"),
                                 end_time=10.0, initial_time_increment=1.0,
                                 constitutive_model=serving.cubic_elastic(), precision=prec)

@@ -136,6 +136,15 @@ typedef struct {
 /* One load increment: predictor, Newton loop with MINRES corrections and
  * convergence tests (small_strain_fft.py:106-211). */
 int mtl_solver_step(mtl_solver* h, const mtl_step_params* p, mtl_step_report* r);
+/* One load increment of several solvers of the same grid and precision at
+ * once -- the same-size requests of a serving stream (no counterpart in the
+ * reference, which solves one material at a time): each solver runs the
+ * Newton loop of mtl_solver_step on its own parameters, and the MINRES solves
+ * of the solvers still iterating run together, one launch of each kernel per
+ * iteration serving all of them.  reports[i] is what mtl_solver_step would
+ * report for handles[i] with params[i]. */
+int mtl_solver_step_batch(mtl_solver* const* handles, int count, const mtl_step_params* params,
+                          mtl_step_report* reports);
 /* Accept the increment (small_strain_fft.py:213-241): old fields <- new,
  * constitutive commit, fluctuation strain scaled by dt_new/dt. */
 int mtl_solver_commit(mtl_solver* h, double fluctuation_scale);

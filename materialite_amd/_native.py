@@ -91,6 +91,8 @@ EXPORTS = {
     "mtl_solver_initialize": (ctypes.c_int, [ctypes.c_void_p]),
     "mtl_solver_step": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(StepParams),
                                        ctypes.POINTER(StepReport)]),
+    "mtl_solver_step_batch": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int,
+                                             ctypes.POINTER(StepParams), ctypes.POINTER(StepReport)]),
     "mtl_solver_commit": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_double]),
     "mtl_solver_rollback": (ctypes.c_int, [ctypes.c_void_p]),
     "mtl_solver_get_field": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
@@ -257,6 +259,18 @@ class NativeSolver:
         rep = StepReport()
         check(self._lib.mtl_solver_step(self._h, ctypes.byref(params), ctypes.byref(rep)))
         return rep
+
+    @staticmethod
+    def step_batch(solvers, params):
+        """One increment of several solvers of one grid and precision together
+        (their MINRES solves share launches); returns one report per solver."""
+        lib = load_library()
+        n = len(solvers)
+        handles = (ctypes.c_void_p * n)(*[s._h.value for s in solvers])
+        ps = (StepParams * n)(*params)
+        reps = (StepReport * n)()
+        check(lib.mtl_solver_step_batch(handles, n, ps, reps))
+        return list(reps)
 
     def commit(self, fluctuation_scale):
         check(self._lib.mtl_solver_commit(self._h, float(fluctuation_scale)))

@@ -39,8 +39,19 @@ __device__ double sum_partials(const double* partials, int count, double* red) {
 template <typename S>
 __global__ void __launch_bounds__(kVBT) k_init(long long nb6, long long n, const double* b,
                                                 RVec<S> r1, RVec<S> r2, work_t<S>* x,
-                                                work_t<S>* w, work_t<S>* w2, double* partials) {
+                                                work_t<S>* w, work_t<S>* w2, double* partials,
+                                                const BatchSlot<S>* slots) {
   using T = work_t<S>;
+  if (slots != nullptr) {  // a batch: the slot's vectors (first iteration's roles)
+    const BatchSlot<S>& sl = slots[blockIdx.y];
+    b = sl.b;
+    r1 = sl.r1;
+    r2 = sl.r2;
+    x = sl.x;
+    w = sl.w1;
+    w2 = sl.w2;
+    partials = sl.partials;
+  }
   __shared__ double red[kVBT / 64];
   double acc = 0.0;
   for (long long i = blockIdx.x * (long long)kVBT + threadIdx.x; i < nb6;
@@ -61,9 +72,16 @@ __global__ void __launch_bounds__(kVBT) k_init(long long nb6, long long n, const
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
+template <typename S>
 __global__ void __launch_bounds__(kSBT) k_start(MinresState* st, const double* partials, int np,
-                                                 double rtol, long long maxiter) {
+                                                 double rtol, long long maxiter,
+                                                 const BatchSlot<S>* slots) {
   __shared__ double red[kSBT / 64];
+  if (slots != nullptr) {
+    st = slots[blockIdx.x].st;
+    partials = slots[blockIdx.x].partials;
+    rtol = slots[blockIdx.x].rtol;
+  }
   const double b2 = sum_partials(partials, np, red);
   if (threadIdx.x != 0) return;
   MinresState z{};
@@ -87,8 +105,14 @@ __global__ void __launch_bounds__(kSBT) k_start(MinresState* st, const double* p
   *st = z;
 }
 
-__global__ void __launch_bounds__(kSBT) k_alpha(MinresState* st, const double* partials, int np) {
+template <typename S>
+__global__ void __launch_bounds__(kSBT) k_alpha(MinresState* st, const double* partials, int np,
+                                                 const BatchSlot<S>* slots) {
   __shared__ double red[kSBT / 64];
+  if (slots != nullptr) {
+    st = slots[blockIdx.x].st;
+    partials = slots[blockIdx.x].partials;
+  }
   if (st->done) return;
   const double a = sum_partials(partials, np, red);
   if (threadIdx.x != 0) return;
@@ -99,8 +123,16 @@ __global__ void __launch_bounds__(kSBT) k_alpha(MinresState* st, const double* p
 // T: arithmetic (FP32 in the FP32 form, FP64 otherwise)
 template <typename S, typename T>
 __global__ void __launch_bounds__(kVBT) k_lanczos(long long n6, const MinresState* st, RVec<S> u,
-                                                   RVec<S> r2, double* partials) {
+                                                   RVec<S> r2, double* partials,
+                                                   const BatchSlot<S>* slots) {
   __shared__ double red[kVBT / 64];
+  if (slots != nullptr) {  // u is the slot's r1 (the matvec's output)
+    const BatchSlot<S>& sl = slots[blockIdx.y];
+    st = sl.st;
+    u = sl.r1;
+    r2 = sl.r2;
+    partials = sl.partials;
+  }
   if (st->done) return;
   const T c = T(st->c_r2);
   double acc = 0.0;
@@ -114,8 +146,14 @@ __global__ void __launch_bounds__(kVBT) k_lanczos(long long n6, const MinresStat
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-__global__ void __launch_bounds__(kSBT) k_givens(MinresState* st, const double* partials, int np) {
+template <typename S>
+__global__ void __launch_bounds__(kSBT) k_givens(MinresState* st, const double* partials, int np,
+                                                  const BatchSlot<S>* slots) {
   __shared__ double red[kSBT / 64];
+  if (slots != nullptr) {
+    st = slots[blockIdx.x].st;
+    partials = slots[blockIdx.x].partials;
+  }
   if (st->done) return;
   const double b2 = sum_partials(partials, np, red);
   if (threadIdx.x != 0) return;
@@ -155,8 +193,19 @@ template <typename S, typename T>
 __global__ void __launch_bounds__(kVBT) k_update(long long n6, const MinresState* st,
                                                   RVec<S> r2old, const work_t<S>* w1,
                                                   const work_t<S>* w2, work_t<S>* wnew,
-                                                  work_t<S>* x, double* partials) {
+                                                  work_t<S>* x, double* partials,
+                                                  const BatchSlot<S>* slots) {
   using W = work_t<S>;
+  if (slots != nullptr) {  // v is the slot's r2 (the Lanczos vector before the swap)
+    const BatchSlot<S>& sl = slots[blockIdx.y];
+    st = sl.st;
+    r2old = sl.r2;
+    w1 = sl.w1;
+    w2 = sl.w2;
+    wnew = sl.wf;
+    x = sl.x;
+    partials = sl.partials;
+  }
   __shared__ double red[kVBT / 64];
   if (st->done) return;
   const T sv = T(st->s_prev), oe = T(st->oldeps), de = T(st->delta), dn = T(st->denom),
@@ -175,8 +224,14 @@ __global__ void __launch_bounds__(kVBT) k_update(long long n6, const MinresState
   if (threadIdx.x == 0) partials[blockIdx.x] = t;
 }
 
-__global__ void __launch_bounds__(kSBT) k_tests(MinresState* st, const double* partials, int np) {
+template <typename S>
+__global__ void __launch_bounds__(kSBT) k_tests(MinresState* st, const double* partials, int np,
+                                                 const BatchSlot<S>* slots) {
   __shared__ double red[kSBT / 64];
+  if (slots != nullptr) {
+    st = slots[blockIdx.x].st;
+    partials = slots[blockIdx.x].partials;
+  }
   if (st->done) return;
   const double x2 = sum_partials(partials, np, red);
   if (threadIdx.x != 0) return;
@@ -233,6 +288,8 @@ Minres<S, A>::~Minres() {
   if (partials_) (void)hipFree(partials_);
   if (state_) (void)hipFree(state_);
   if (host_state_) (void)hipHostFree(host_state_);
+  if (slots_dev_) (void)hipFree(slots_dev_);
+  if (slots_host_) (void)hipHostFree(slots_host_);
 }
 
 template <typename S, typename A>
@@ -246,9 +303,10 @@ int Minres<S, A>::solve(const double* b, const W* tangent, const Vec6& mask, dou
   W* wf = w_[2];
   const int vb = vec_blocks(n6_);
   hipLaunchKernelGGL(k_init<S>, dim3(vb), dim3(kVBT), 0, s, n6_, sp_.grid().n, b, r1, r2, x_, w2,
-                     wf, partials_);
+                     wf, partials_, static_cast<const BatchSlot<S>*>(nullptr));
   MTL_LAUNCH_CHECK();
-  hipLaunchKernelGGL(k_start, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb, rtol, maxiter);
+  hipLaunchKernelGGL(k_start<S>, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb, rtol, maxiter,
+                     static_cast<const BatchSlot<S>*>(nullptr));
   MTL_LAUNCH_CHECK();
   // at itn 1 the previous w's are zero: w1 := wf (zeroed), w2 (zeroed)
   std::swap(w1, wf);
@@ -260,18 +318,23 @@ int Minres<S, A>::solve(const double* b, const W* tangent, const Vec6& mask, dou
   while (true) {
     for (int it = 0; it < chunk; ++it) {
       const int np = sp_.matvec_minres(r2, tangent, r1, mask, state_, partials_, s);
-      profiler().launch(kKMinresScalar, k_alpha, dim3(1), dim3(kSBT), 0, s, state_, partials_, np);
+      profiler().launch(kKMinresScalar, k_alpha<S>, dim3(1), dim3(kSBT), 0, s, state_, partials_, np,
+                        static_cast<const BatchSlot<S>*>(nullptr));
       MTL_LAUNCH_CHECK();
-      profiler().launch(kKMinresVector, k_lanczos<S, A>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, r2, partials_);
+      profiler().launch(kKMinresVector, k_lanczos<S, A>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, r2, partials_,
+                        static_cast<const BatchSlot<S>*>(nullptr));
       MTL_LAUNCH_CHECK();
-      profiler().launch(kKMinresScalar, k_givens, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb);
+      profiler().launch(kKMinresScalar, k_givens<S>, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb,
+                        static_cast<const BatchSlot<S>*>(nullptr));
       MTL_LAUNCH_CHECK();
       // Lanczos roles rotate: r1 <- r2 (old), r2 <- y (stored in old r1)
       std::swap(r1, r2);
       // v_k = r2_old / beta_k now lives in r1
-      profiler().launch(kKMinresVector, k_update<S, A>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, w1, w2, wf, x_, partials_);
+      profiler().launch(kKMinresVector, k_update<S, A>, dim3(vb), dim3(kVBT), 0, s, n6_, state_, r1, w1, w2, wf, x_, partials_,
+                        static_cast<const BatchSlot<S>*>(nullptr));
       MTL_LAUNCH_CHECK();
-      profiler().launch(kKMinresScalar, k_tests, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb);
+      profiler().launch(kKMinresScalar, k_tests<S>, dim3(1), dim3(kSBT), 0, s, state_, partials_, vb,
+                        static_cast<const BatchSlot<S>*>(nullptr));
       MTL_LAUNCH_CHECK();
       // w roles rotate: w1 <- w2, w2 <- new w, free slot <- old w1
       W* old_w1 = w1;
@@ -302,6 +365,112 @@ int Minres<S, A>::solve(const double* b, const W* tangent, const Vec6& mask, dou
   }
   if (istop_out) *istop_out = host_state_->istop;
   return host_state_->itn;
+}
+
+template <typename S, typename A>
+void Minres<S, A>::solve_batch(const std::vector<BatchItem>& items, long long maxiter,
+                               hipStream_t s, int* itns) {
+  const int nb = int(items.size());
+  if (nb == 0) return;
+  Minres& lead = *items[0].m;
+  for (const auto& it : items)
+    if (it.m->n6_ != lead.n6_) throw Error("a MINRES batch needs solvers of one grid");
+  // slot tables of the six role phases (Lanczos roles alternate, w roles
+  // rotate with period 3), kept by the first solver of the batch
+  if (lead.slot_cap_ < nb) {
+    if (lead.slots_dev_) MTL_HIP(hipFree(lead.slots_dev_));
+    if (lead.slots_host_) MTL_HIP(hipHostFree(lead.slots_host_));
+    lead.slots_dev_ = nullptr;
+    lead.slots_host_ = nullptr;
+    MTL_HIP(hipMalloc(&lead.slots_dev_, 6 * size_t(nb) * sizeof(BatchSlot<S>)));
+    MTL_HIP(hipHostMalloc(&lead.slots_host_, 6 * size_t(nb) * sizeof(BatchSlot<S>),
+                          hipHostMallocDefault));
+    lead.slot_cap_ = nb;
+  }
+  for (int ph = 0; ph < 6; ++ph)
+    for (int i = 0; i < nb; ++i) {
+      const BatchItem& it = items[i];
+      Minres& m = *it.m;
+      BatchSlot<S>& sl = lead.slots_host_[ph * nb + i];
+      sl.spec = m.sp_.spectrum();
+      sl.tangent = it.tangent;
+      sl.r1 = m.lanczos_[ph % 2];
+      sl.r2 = m.lanczos_[(ph + 1) % 2];
+      sl.maskbits = Spectral<S, A>::mask_bits(it.mask);
+      sl.st = m.state_;
+      sl.partials = m.partials_;
+      sl.w1 = m.w_[(2 - ph % 3 + 3) % 3];
+      sl.w2 = m.w_[(1 - ph % 3 + 3) % 3];
+      sl.wf = m.w_[(3 - ph % 3) % 3];
+      sl.x = m.x_;
+      sl.b = it.b;
+      sl.rtol = it.rtol;
+    }
+  MTL_HIP(hipMemcpyAsync(lead.slots_dev_, lead.slots_host_, 6 * size_t(nb) * sizeof(BatchSlot<S>),
+                         hipMemcpyHostToDevice, s));
+  const BatchSlot<S>* tab = lead.slots_dev_;
+  const int vb = vec_blocks(lead.n6_);
+  hipLaunchKernelGGL(k_init<S>, dim3(vb, nb), dim3(kVBT), 0, s, lead.n6_, lead.sp_.grid().n,
+                     static_cast<const double*>(nullptr), RVec<S>{}, RVec<S>{},
+                     static_cast<work_t<S>*>(nullptr), static_cast<work_t<S>*>(nullptr),
+                     static_cast<work_t<S>*>(nullptr), static_cast<double*>(nullptr), tab);
+  MTL_LAUNCH_CHECK();
+  hipLaunchKernelGGL(k_start<S>, dim3(nb), dim3(kSBT), 0, s, static_cast<MinresState*>(nullptr),
+                     static_cast<const double*>(nullptr), vb, 0.0, maxiter, tab);
+  MTL_LAUNCH_CHECK();
+  int chunk = 8;
+  long long launched = 0;
+  std::vector<int> prev_itn(nb, 0);
+  std::vector<double> prev_test1(nb, 0.0);
+  while (true) {
+    for (int k = 0; k < chunk; ++k) {
+      const BatchSlot<S>* t = tab + (launched % 6) * nb;
+      const int np = lead.sp_.matvec_batch(t, nb, s);
+      profiler().launch(kKMinresScalar, k_alpha<S>, dim3(nb), dim3(kSBT), 0, s,
+                        static_cast<MinresState*>(nullptr), static_cast<const double*>(nullptr), np, t);
+      MTL_LAUNCH_CHECK();
+      profiler().launch(kKMinresVector, k_lanczos<S, A>, dim3(vb, nb), dim3(kVBT), 0, s, lead.n6_,
+                        static_cast<const MinresState*>(nullptr), RVec<S>{}, RVec<S>{},
+                        static_cast<double*>(nullptr), t);
+      MTL_LAUNCH_CHECK();
+      profiler().launch(kKMinresScalar, k_givens<S>, dim3(nb), dim3(kSBT), 0, s,
+                        static_cast<MinresState*>(nullptr), static_cast<const double*>(nullptr), vb, t);
+      MTL_LAUNCH_CHECK();
+      profiler().launch(kKMinresVector, k_update<S, A>, dim3(vb, nb), dim3(kVBT), 0, s, lead.n6_,
+                        static_cast<const MinresState*>(nullptr), RVec<S>{},
+                        static_cast<const work_t<S>*>(nullptr), static_cast<const work_t<S>*>(nullptr),
+                        static_cast<work_t<S>*>(nullptr), static_cast<work_t<S>*>(nullptr),
+                        static_cast<double*>(nullptr), t);
+      MTL_LAUNCH_CHECK();
+      profiler().launch(kKMinresScalar, k_tests<S>, dim3(nb), dim3(kSBT), 0, s,
+                        static_cast<MinresState*>(nullptr), static_cast<const double*>(nullptr), vb, t);
+      MTL_LAUNCH_CHECK();
+      ++launched;
+    }
+    for (const auto& it : items)
+      MTL_HIP(hipMemcpyAsync(it.m->host_state_, it.m->state_, sizeof(MinresState),
+                             hipMemcpyDeviceToHost, s));
+    stream_sync(s);
+    bool all_done = true;
+    chunk = 2;
+    for (int i = 0; i < nb; ++i) {
+      const MinresState& z = *items[i].m->host_state_;
+      if (z.done) continue;
+      all_done = false;
+      // the slowest system sizes the next chunk (SciPy test1's observed decay)
+      int need = 8;
+      if (prev_itn[i] > 0 && z.itn > prev_itn[i] && z.test1 > 0.0 && prev_test1[i] > z.test1) {
+        const double rate = std::pow(z.test1 / prev_test1[i], 1.0 / double(z.itn - prev_itn[i]));
+        const double est = std::log(z.rtol / z.test1) / std::log(rate);
+        if (std::isfinite(est)) need = int(std::min(64.0, std::max(2.0, std::ceil(est) + 1.0)));
+      }
+      chunk = std::max(chunk, need);
+      prev_itn[i] = z.itn;
+      prev_test1[i] = z.test1;
+    }
+    if (all_done || launched >= maxiter) break;
+  }
+  for (int i = 0; i < nb; ++i) itns[i] = items[i].m->host_state_->itn;
 }
 
 template class Minres<float>;

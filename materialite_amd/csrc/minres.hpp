@@ -1,6 +1,8 @@
 // Device MINRES for the Galerkin system  Ghat * (D : x) = b.
 #pragma once
 
+#include <vector>
+
 #include "spectral.hpp"
 
 namespace mtl {
@@ -23,6 +25,22 @@ class Minres {
             hipStream_t s, int* istop_out);
 
   const W* x() const { return x_; }
+
+  // One system of a batch: this solver, its right-hand side and tangent.
+  struct BatchItem {
+    Minres* m;
+    const double* b;
+    const W* tangent;
+    Vec6 mask;
+    double rtol;
+  };
+  // Solves the systems of several solvers of the same grid and form at once
+  // (x0 = 0 each): every iteration is one launch of each kernel for all of
+  // them, with per-system scalars, stopping tests and iteration counts
+  // (itns[i]); a system that has stopped costs its workgroups one flag read.
+  // Each result is the one solve() gives for the same system.
+  static void solve_batch(const std::vector<BatchItem>& items, long long maxiter, hipStream_t s,
+                          int* itns);
   // Host copy of the last state (valid after solve()).
   const MinresState& state() const { return *host_state_; }
 
@@ -37,6 +55,10 @@ class Minres {
   int partial_cap_ = 0;
   MinresState* state_ = nullptr;
   MinresState* host_state_ = nullptr;  // pinned
+  // batch slot tables (six iteration phases x capacity), device and pinned host
+  BatchSlot<S>* slots_dev_ = nullptr;
+  BatchSlot<S>* slots_host_ = nullptr;
+  int slot_cap_ = 0;
 };
 
 }  // namespace mtl

@@ -478,136 +478,204 @@ class Solver final : public SolverBase {
     sp64_.project(sig_, b_, applied, -1.0, mask, s_);
   }
 
+  // One load increment (small_strain_fft.py:106-211) as resumable phases, so
+  // that the MINRES solves of several same-size solvers can run together:
+  // step_begin (predictor), then while !step_finished(): step_pre_minres and,
+  // when it returns true, one MINRES solve followed by step_post_minres.
   void step(const mtl_step_params& P, mtl_step_report& R) override {
+    step_begin(P, R);
+    while (!ss_.finished) {
+      if (!step_pre_minres(R)) continue;
+      int istop = 0;
+      const int itn = minres_->solve(b_, D_, ss_.mask, ss_.lin_tol, 5LL * 6LL * n_, s_, &istop);
+      if (debug_enabled())
+        std::fprintf(stderr, "[mtl] step %d newton %d: minres %d iterations (istop %d)\n",
+                     ss_.P.time_step_id, ss_.iteration, itn, istop);
+      step_post_minres(itn, R);
+    }
+    step_end(R);
+  }
+
+  // The same increment for several solvers of one grid and form at once.
+  static void step_batch(const std::vector<Solver*>& hs, const mtl_step_params* P,
+                         mtl_step_report* R) {
+    const int nb = int(hs.size());
+    for (int i = 0; i < nb; ++i) hs[i]->step_begin(P[i], R[i]);
+    std::vector<typename Minres<S, A>::BatchItem> items;
+    std::vector<int> who, itns;
+    while (true) {
+      items.clear();
+      who.clear();
+      bool any = false;
+      for (int i = 0; i < nb; ++i) {
+        Solver& h = *hs[i];
+        if (h.ss_.finished) continue;
+        any = true;
+        if (h.step_pre_minres(R[i])) {
+          items.push_back({h.minres_.get(), h.b_, h.D_, h.ss_.mask, h.ss_.lin_tol});
+          who.push_back(i);
+        }
+      }
+      if (!any) break;
+      if (items.empty()) continue;
+      itns.assign(items.size(), 0);
+      // every solver's stream has drained (its residual norms were read on
+      // the host); the batch runs on the first solver's stream
+      Minres<S, A>::solve_batch(items, 5LL * 6LL * hs[0]->n_, hs[who[0]]->s_, itns.data());
+      for (size_t k = 0; k < who.size(); ++k) hs[who[k]]->step_post_minres(itns[k], R[who[k]]);
+    }
+    for (int i = 0; i < nb; ++i) hs[i]->step_end(R[i]);
+  }
+
+  const Grid& grid() const { return sp64_.grid(); }
+
+ private:
+  struct StepState {
+    mtl_step_params P;
+    Vec6 de, ds, applied, mask;
+    double dt = 0, lin_tol = 0, eq_err = 0, vm = 0, old_eq = 0;
+    int iteration = 0, max_constit = 0;
+    bool converged = false, finished = false;
+  };
+  StepState ss_;
+
+  void step_begin(const mtl_step_params& P, mtl_step_report& R) {
     std::memset(&R, 0, sizeof(R));
-    Vec6 de, ds, applied, mask, pred{};
-    std::memcpy(de.v, P.strain_increment, sizeof(de.v));
-    std::memcpy(ds.v, P.stress_increment, sizeof(ds.v));
-    std::memcpy(applied.v, P.applied_stress, sizeof(applied.v));
-    std::memcpy(mask.v, P.stress_mask, sizeof(mask.v));
-    const double dt = P.time_increment;
+    ss_ = StepState{};
+    ss_.P = P;
+    Vec6 pred{};
+    std::memcpy(ss_.de.v, P.strain_increment, sizeof(ss_.de.v));
+    std::memcpy(ss_.ds.v, P.stress_increment, sizeof(ss_.ds.v));
+    std::memcpy(ss_.applied.v, P.applied_stress, sizeof(ss_.applied.v));
+    std::memcpy(ss_.mask.v, P.stress_mask, sizeof(ss_.mask.v));
+    ss_.dt = P.time_increment;
     // predictor (small_strain_fft.py:117-123)
     bool all_small = true;
-    for (double v : ds.v) all_small = all_small && (v < 1.0e-14);
+    for (double v : ss_.ds.v) all_small = all_small && (v < 1.0e-14);
     if (!all_small) {
       double dm[36], dinv[36];
       tangent_mean(dm);
       if (!inverse6_host(dm, dinv)) throw Error("mean tangent is singular");
       for (int a = 0; a < 6; ++a) {
         double acc = 0.0;
-        for (int b = 0; b < 6; ++b) acc += dinv[a * 6 + b] * ds.v[b];
+        for (int b = 0; b < 6; ++b) acc += dinv[a * 6 + b] * ss_.ds.v[b];
         pred.v[a] = acc;
       }
     }
     const int nb = grid_for(n_);
-    hipLaunchKernelGGL(k_predictor, dim3(nb), dim3(kNBT), 0, s_, n_, eps_old_, fluct_, eps_, de,
+    hipLaunchKernelGGL(k_predictor, dim3(nb), dim3(kNBT), 0, s_, n_, eps_old_, fluct_, eps_, ss_.de,
                        pred, P.reset_fluctuation);
     MTL_LAUNCH_CHECK();
     eps_stale_ = false;
     hipLaunchKernelGGL(k_guess<K>, dim3(nb), dim3(kNBT), 0, s_, n_, P.time_step_id == 0 ? 1 : 0,
-                       sig_old_, cur_D(), ds, de, guess_);
+                       sig_old_, cur_D(), ss_.ds, ss_.de, guess_);
     MTL_LAUNCH_CHECK();
     guess_is_stress_ = false;
-
-    const int kMaxNewton = 100;  // small_strain_fft.py:125
-    const long long maxiter = 5LL * 6LL * n_;
-    // Mixed forms: FP32 Krylov solves cannot honour relative tolerances much
-    // below 1e-5 (1e-6 with FP64 FFT arithmetic) on the singular pore
-    // systems, so the inner tolerance is floored there.  The Newton residual,
-    // the equilibrium test and the strain/stress fields stay FP64, so each
-    // Newton iteration is an FP64-residual refinement step and the converged
-    // state is the FP64 one up to the requested global tolerance.
-    constexpr bool kMixed = !std::is_same<K, double>::value;
+    // The FP32 form's Krylov solves cannot honour relative tolerances much
+    // below 1e-5 on the singular pore systems, so its inner tolerance is
+    // floored there.  The Newton residual, the equilibrium test and the
+    // strain/stress fields stay FP64 in every form, so each Newton iteration
+    // is an FP64-residual refinement step.
     constexpr double kFloor = std::is_same<S, float>::value ? kMixedKrylovFloor : 0.0;
-    const double lin_tol = std::max(P.linear_solver_tolerance, kFloor);
-    R.linear_solver_tolerance_used = lin_tol;
-    bool converged = false;
-    int iteration = 0;
-    int max_constit = 0;
-    double eq_err = 0.0, vm = 0.0, old_eq = 0.0;
-    while (!converged) {
-      ++iteration;
-      if (iteration == 1) {
-        const int it = constitutive_all(eps_, guess_, dt, sig_);
-        sig_stale_ = false;
-        if (!it) {
-          restore_tangent();
-          R.constitutive_failed = 1;
-          break;
-        }
-        residual(applied, mask);
-        const Norms nm = norms(false);
-        vm = von_mises_of_mean(nm.mean);
-        eq_err = nm.maxb / vm;
-        // the next constitutive call's guess is this stress; it rewrites sig_
-        std::swap(guess_, sig_);
-        guess_is_stress_ = true;
-        max_constit = std::max(max_constit, it);
-      }
-      old_eq = std::max(eq_err, P.global_tolerance);
-      int istop = 0;
-      const int itn = minres_->solve(b_, D_, mask, lin_tol, maxiter, s_, &istop);
-      R.krylov_iterations += itn;
-      if (debug_enabled())
-        std::fprintf(stderr, "[mtl] step %d newton %d: minres %d iterations (istop %d)\n",
-                     P.time_step_id, iteration, itn, istop);
-      if (kMixed) {
-        // The exact correction is compatible, with zero mean in every
-        // strain-controlled component (Ghat[0] = diag(mask)); the FP32 FFTs
-        // leave a ~1e-7 relative mean there, removed here in FP64 (on the
-        // device: no host round trip).
-        hipLaunchKernelGGL(k_sum6<K>, dim3(nb), dim3(kNBT), 0, s_, n_, minres_->x(), partials_);
-        MTL_LAUNCH_CHECK();
-        hipLaunchKernelGGL(k_mean_shift, dim3(1), dim3(kNBT), 0, s_, nb, partials_, n_, mask,
-                           shift_);
-        MTL_LAUNCH_CHECK();
-      }
-      hipLaunchKernelGGL(k_add_correction<K>, dim3(nb), dim3(kNBT), 0, s_, n_, minres_->x(), eps_,
-                         fluct_, kMixed ? shift_ : static_cast<const double*>(nullptr));
-      MTL_LAUNCH_CHECK();
-      const int it = constitutive_all(eps_, guess_, dt, sig_);
-      sig_stale_ = false;
-      if (!it) {
-        // sig_ is left as the last accepted stress (the failed call's guess)
-        if (guess_is_stress_) std::swap(guess_, sig_);
-        restore_tangent();
-        R.constitutive_failed = 1;
-        break;
-      }
-      max_constit = std::max(max_constit, it);
-      residual(applied, mask);
-      const Norms nm = norms(true);
-      vm = von_mises_of_mean(nm.mean);
-      const double time_avg = (P.summed_von_mises_stress + vm) / double(P.time_step_id + 1);
-      if (vm / time_avg < 1.0e-12) vm = time_avg;
-      eq_err = nm.maxb / vm;
-      double max_ds = std::sqrt(nm.maxdev2) * std::sqrt(1.5);
-      if (max_ds / vm < 1.0e-12) max_ds = vm;
-      const double strain_err = std::sqrt(nm.maxdx2) / max_ds;
-      const double projected = strain_err * eq_err / old_eq;
-      if (eq_err < P.global_tolerance &&
-          (strain_err < P.strain_correction_tolerance ||
-           projected < P.strain_correction_tolerance || P.time_step_id == 0 || eq_err < 1e-8))
-        converged = true;
-      if (debug_enabled())
-        std::fprintf(stderr, "[mtl]   eq_err %.3e strain_err %.3e projected %.3e -> %s\n", eq_err,
-                     strain_err, projected, converged ? "converged" : "continue");
-      if (!converged && iteration < kMaxNewton) {
-        std::swap(guess_, sig_);  // as above: sig_ is rewritten before it is read
-        guess_is_stress_ = true;
-      }
-      R.scaled_strain_error = strain_err;
-      R.projected_scaled_strain_error = projected;
-      if (iteration == kMaxNewton && !converged) break;
-    }
-    profiler().flush();
-    R.converged = converged ? 1 : 0;
-    R.newton_iterations = iteration;
-    R.max_constitutive_iterations = max_constit;
-    R.mean_von_mises_stress = vm;
-    R.equilibrium_error = eq_err;
+    ss_.lin_tol = std::max(P.linear_solver_tolerance, kFloor);
+    R.linear_solver_tolerance_used = ss_.lin_tol;
   }
 
+  // Next Newton iteration up to its MINRES solve: false when the increment
+  // ended instead (the first constitutive update failed).
+  bool step_pre_minres(mtl_step_report& R) {
+    StepState& q = ss_;
+    ++q.iteration;
+    if (q.iteration == 1) {
+      const int it = constitutive_all(eps_, guess_, q.dt, sig_);
+      sig_stale_ = false;
+      if (!it) {
+        restore_tangent();
+        R.constitutive_failed = 1;
+        q.finished = true;
+        return false;
+      }
+      residual(q.applied, q.mask);
+      const Norms nm = norms(false);
+      q.vm = von_mises_of_mean(nm.mean);
+      q.eq_err = nm.maxb / q.vm;
+      // the next constitutive call's guess is this stress; it rewrites sig_
+      std::swap(guess_, sig_);
+      guess_is_stress_ = true;
+      q.max_constit = std::max(q.max_constit, it);
+    }
+    q.old_eq = std::max(q.eq_err, q.P.global_tolerance);
+    return true;
+  }
+
+  // The rest of the Newton iteration after its MINRES solve of itn iterations.
+  void step_post_minres(int itn, mtl_step_report& R) {
+    StepState& q = ss_;
+    const mtl_step_params& P = q.P;
+    R.krylov_iterations += itn;
+    constexpr bool kMixed = !std::is_same<K, double>::value;
+    const int nb = grid_for(n_);
+    if (kMixed) {
+      // The exact correction is compatible, with zero mean in every
+      // strain-controlled component (Ghat[0] = diag(mask)); FP32 rounding
+      // leaves a small relative mean there, removed here in FP64 (on the
+      // device: no host round trip).
+      hipLaunchKernelGGL(k_sum6<K>, dim3(nb), dim3(kNBT), 0, s_, n_, minres_->x(), partials_);
+      MTL_LAUNCH_CHECK();
+      hipLaunchKernelGGL(k_mean_shift, dim3(1), dim3(kNBT), 0, s_, nb, partials_, n_, q.mask, shift_);
+      MTL_LAUNCH_CHECK();
+    }
+    hipLaunchKernelGGL(k_add_correction<K>, dim3(nb), dim3(kNBT), 0, s_, n_, minres_->x(), eps_,
+                       fluct_, kMixed ? shift_ : static_cast<const double*>(nullptr));
+    MTL_LAUNCH_CHECK();
+    const int it = constitutive_all(eps_, guess_, q.dt, sig_);
+    sig_stale_ = false;
+    if (!it) {
+      // sig_ is left as the last accepted stress (the failed call's guess)
+      if (guess_is_stress_) std::swap(guess_, sig_);
+      restore_tangent();
+      R.constitutive_failed = 1;
+      q.finished = true;
+      return;
+    }
+    q.max_constit = std::max(q.max_constit, it);
+    residual(q.applied, q.mask);
+    const Norms nm = norms(true);
+    q.vm = von_mises_of_mean(nm.mean);
+    const double time_avg = (P.summed_von_mises_stress + q.vm) / double(P.time_step_id + 1);
+    if (q.vm / time_avg < 1.0e-12) q.vm = time_avg;
+    q.eq_err = nm.maxb / q.vm;
+    double max_ds = std::sqrt(nm.maxdev2) * std::sqrt(1.5);
+    if (max_ds / q.vm < 1.0e-12) max_ds = q.vm;
+    const double strain_err = std::sqrt(nm.maxdx2) / max_ds;
+    const double projected = strain_err * q.eq_err / q.old_eq;
+    if (q.eq_err < P.global_tolerance &&
+        (strain_err < P.strain_correction_tolerance ||
+         projected < P.strain_correction_tolerance || P.time_step_id == 0 || q.eq_err < 1e-8))
+      q.converged = true;
+    if (debug_enabled())
+      std::fprintf(stderr, "[mtl]   eq_err %.3e strain_err %.3e projected %.3e -> %s\n", q.eq_err,
+                   strain_err, projected, q.converged ? "converged" : "continue");
+    constexpr int kMaxNewton = 100;  // small_strain_fft.py:125
+    if (!q.converged && q.iteration < kMaxNewton) {
+      std::swap(guess_, sig_);  // as above: sig_ is rewritten before it is read
+      guess_is_stress_ = true;
+    }
+    R.scaled_strain_error = strain_err;
+    R.projected_scaled_strain_error = projected;
+    if (q.converged || q.iteration == kMaxNewton) q.finished = true;
+  }
+
+  void step_end(mtl_step_report& R) {
+    profiler().flush();
+    R.converged = ss_.converged ? 1 : 0;
+    R.newton_iterations = ss_.iteration;
+    R.max_constitutive_iterations = ss_.max_constit;
+    R.mean_von_mises_stress = ss_.vm;
+    R.equilibrium_error = ss_.eq_err;
+  }
+
+ public:
   void commit(double scale) override {
     ms_.commit(s_);
     // the current stress, strain and tangent become the committed ones by
@@ -862,6 +930,25 @@ int bad(const char* msg) {
 }
 }  // namespace
 
+namespace {
+template <typename Sv>
+bool try_step_batch(mtl_solver* const* hs, int count, const mtl_step_params* p,
+                    mtl_step_report* r) {
+  std::vector<Sv*> v;
+  for (int i = 0; i < count; ++i) {
+    auto* s = dynamic_cast<Sv*>(hs[i]->impl.get());
+    if (s == nullptr) return false;
+    v.push_back(s);
+  }
+  const mtl::Grid& g = v[0]->grid();
+  for (auto* s : v)
+    if (s->grid().nx != g.nx || s->grid().ny != g.ny || s->grid().nz != g.nz)
+      throw mtl::Error("a step batch needs solvers of one grid");
+  Sv::step_batch(v, p, r);
+  return true;
+}
+}  // namespace
+
 extern "C" {
 
 const char* mtl_last_error(void) { return mtl::g_last_error.c_str(); }
@@ -966,6 +1053,22 @@ int mtl_solver_initialize(mtl_solver* h) {
 int mtl_solver_step(mtl_solver* h, const mtl_step_params* p, mtl_step_report* r) {
   if (!h || !p || !r) return bad("null argument");
   return guarded([&] { h->impl->step(*p, *r); });
+}
+
+int mtl_solver_step_batch(mtl_solver* const* handles, int count, const mtl_step_params* params,
+                          mtl_step_report* reports) {
+  if (!handles || !params || !reports || count < 1) return bad("null argument or empty batch");
+  for (int i = 0; i < count; ++i)
+    if (!handles[i]) return bad("null handle in batch");
+  return guarded([&] {
+    for (int i = 0; i < count; ++i)
+      for (int j = 0; j < i; ++j)
+        if (handles[i] == handles[j]) throw mtl::Error("a handle appears twice in a step batch");
+    if (try_step_batch<mtl::Solver<double>>(handles, count, params, reports)) return;
+    if (try_step_batch<mtl::Solver<float>>(handles, count, params, reports)) return;
+    if (try_step_batch<mtl::Solver<mtl::f48, double>>(handles, count, params, reports)) return;
+    throw mtl::Error("a step batch needs grid solvers of one precision");
+  });
 }
 
 int mtl_solver_commit(mtl_solver* h, double fluctuation_scale) {

@@ -74,9 +74,18 @@ template <typename T, typename S, int MODE, typename In, bool BIG>
 __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan_g, const cplx<T>* twz,
                                                      int P, In in, const work_t<S>* tangent,
                                                      Vec6 a, double sgn,
-                                                     const MinresState* st, CVec<S> spec) {
-  if constexpr (MODE == kModeMatvec)
+                                                     const MinresState* st, CVec<S> spec,
+                                                     const BatchSlot<S>* slots) {
+  if constexpr (MODE == kModeMatvec) {
+    if (slots != nullptr) {
+      const BatchSlot<S>& sl = slots[blockIdx.y];
+      in = sl.r2;
+      tangent = sl.tangent;
+      st = sl.st;
+      spec = sl.spec;
+    }
     if (st->done) return;
+  }
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -136,7 +145,12 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan
 template <typename T, typename S, bool BIG>
 __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g, const cplx<T>* twy, int TY,
                                                   int ntiles, bool inv,
-                                                  const MinresState* st, CVec<S> spec) {
+                                                  const MinresState* st, CVec<S> spec,
+                                                  const BatchSlot<S>* slots) {
+  if (slots != nullptr) {
+    st = slots[blockIdx.y].st;
+    spec = slots[blockIdx.y].spec;
+  }
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
@@ -228,7 +242,13 @@ template <typename T, typename S, bool BIG>
 __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
                                                   const T* tanz, int maskbits,
-                                                  const MinresState* st, CVec<S> spec) {
+                                                  const MinresState* st, CVec<S> spec,
+                                                  const BatchSlot<S>* slots) {
+  if (slots != nullptr) {
+    st = slots[blockIdx.y].st;
+    spec = slots[blockIdx.y].spec;
+    maskbits = slots[blockIdx.y].maskbits;
+  }
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
@@ -334,9 +354,15 @@ template <typename T, typename S, bool PAIRS, bool BIG>
 __global__ void __launch_bounds__(kXBT) k_x_ghat_streamed(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
                                                   const T* tanz, int maskbits,
-                                                  const MinresState* st, CVec<S> spec) {
+                                                  const MinresState* st, CVec<S> spec,
+                                                  const BatchSlot<S>* slots) {
   static_assert(!PAIRS || (sizeof(T) == 4 && std::is_same<S, float>::value),
                 "pairs of complex values are 16 B in FP32 only");
+  if (slots != nullptr) {
+    st = slots[blockIdx.y].st;
+    spec = slots[blockIdx.y].spec;
+    maskbits = slots[blockIdx.y].maskbits;
+  }
   if (st != nullptr && st->done) return;
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
@@ -444,9 +470,19 @@ template <typename T, typename S, int MODE, bool BIG>
 __global__ void __launch_bounds__(kZBT) k_z_inverse(Grid g, const AxisPlan* plan_g, const cplx<T>* twz,
                                                      int P, CVec<S> spec, double* out,
                                                      RVec<S> r1, RVec<S> r2,
-                                                     const MinresState* st, double* partials) {
-  if constexpr (MODE == kModeMatvec)
+                                                     const MinresState* st, double* partials,
+                                                     const BatchSlot<S>* slots) {
+  if constexpr (MODE == kModeMatvec) {
+    if (slots != nullptr) {
+      const BatchSlot<S>& sl = slots[blockIdx.y];
+      spec = sl.spec;
+      r1 = sl.r1;
+      r2 = sl.r2;
+      st = sl.st;
+      partials = sl.partials;
+    }
     if (st->done) return;
+  }
   extern __shared__ unsigned char smem_raw[];
   const AxisPlan* plan = plan_g;
   cplx<T>* tw = reinterpret_cast<cplx<T>*>(smem_raw);
@@ -596,6 +632,11 @@ void allow_lds(K kernel, size_t bytes) {
 }
 
 }  // namespace
+
+template <typename S, typename A>
+int Spectral<S, A>::mask_bits(const Vec6& mask) {
+  return ::mtl::mask_bits(mask);
+}
 
 template <typename S, typename A>
 void Spectral<S, A>::plan(int nx, int ny, int nz) {
@@ -756,13 +797,13 @@ template <typename S, typename A>
 template <int MODE, typename In>
 void Spectral<S, A>::run(In in, const W* tangent, double* out, RVec<S> r1, const Vec6& a,
                          double sgn, const Vec6& mask, const MinresState* st, double* partials,
-                         hipStream_t s) {
+                         hipStream_t s, const BatchSlot<S>* slots, int nb) {
   const int P = pz_lines_;
   const int zblocks = zinv_blocks();
   const size_t zlds = (size_t(g_.nz) + 2 * size_t(zbuf_elems(P, g_.nz, g_.nzc))) * sizeof(cplx<A>);
   auto kzf = big_z_ ? k_z_forward<A, S, MODE, In, true> : k_z_forward<A, S, MODE, In, false>;
   allow_lds(kzf, zlds);
-  profiler().launch(MODE == kModeMatvec ? kKZForward : residual_id(kKZForward), kzf, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, in, tangent, a, sgn, st, spec_);
+  profiler().launch(MODE == kModeMatvec ? kKZForward : residual_id(kKZForward), kzf, dim3(zblocks, nb), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, in, tangent, a, sgn, st, spec_, slots);
   MTL_LAUNCH_CHECK();
 
   const MinresState* stp = MODE == kModeMatvec ? st : nullptr;
@@ -770,7 +811,7 @@ void Spectral<S, A>::run(In in, const W* tangent, double* out, RVec<S> r1, const
   const size_t ylds = (size_t(g_.ny) + 2 * size_t(ty_) * g_.ny) * sizeof(cplx<A>);
   auto ky = big_y_ ? k_y_pass<A, S, true> : k_y_pass<A, S, false>;
   allow_lds(ky, ylds);
-  profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, false, stp, spec_);
+  profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles, nb), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, false, stp, spec_, slots);
   MTL_LAUNCH_CHECK();
 
   const int xtiles = ceil_div(g_.nzc, tx_);
@@ -790,17 +831,17 @@ void Spectral<S, A>::run(In in, const W* tangent, double* out, RVec<S> r1, const
     }
   }
   allow_lds(kx, xlds);
-  profiler().launch(MODE == kModeMatvec ? kKXGhat : residual_id(kKXGhat), kx, dim3(g_.ny * xtiles), dim3(kXBT), xlds, s, g_, pxd_, twx_, tx_, xtiles, tanx_, tany_, tanz_, mask_bits(mask), stp, spec_);
+  profiler().launch(MODE == kModeMatvec ? kKXGhat : residual_id(kKXGhat), kx, dim3(g_.ny * xtiles, nb), dim3(kXBT), xlds, s, g_, pxd_, twx_, tx_, xtiles, tanx_, tany_, tanz_, mask_bits(mask), stp, spec_, slots);
   MTL_LAUNCH_CHECK();
 
-  profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, true, stp, spec_);
+  profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles, nb), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, true, stp, spec_, slots);
   MTL_LAUNCH_CHECK();
 
   RVec<S> r2{};
   if constexpr (MODE == kModeMatvec) r2 = in;
   auto kzi = big_z_ ? k_z_inverse<A, S, MODE, true> : k_z_inverse<A, S, MODE, false>;
   allow_lds(kzi, zlds);
-  profiler().launch(MODE == kModeMatvec ? kKZInverse : residual_id(kKZInverse), kzi, dim3(zblocks), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, spec_, out, r1, r2, st, partials);
+  profiler().launch(MODE == kModeMatvec ? kKZInverse : residual_id(kKZInverse), kzi, dim3(zblocks, nb), dim3(kZBT), zlds, s, g_, pzd_, twz_, P, spec_, out, r1, r2, st, partials, slots);
   MTL_LAUNCH_CHECK();
 }
 
@@ -815,6 +856,16 @@ int Spectral<S, A>::matvec_minres(RVec<S> r2, const W* tangent, RVec<S> r1, cons
                                   const MinresState* st, double* partials, hipStream_t s) {
   Vec6 zero{};
   run<kModeMatvec, RVec<S>>(r2, tangent, nullptr, r1, zero, 1.0, mask, st, partials, s);
+  return zinv_blocks();
+}
+
+template <typename S, typename A>
+int Spectral<S, A>::matvec_batch(const BatchSlot<S>* slots, int nb, hipStream_t s) {
+  // per-solve arguments come from the slots; the mask only picks the kernels'
+  // shared arguments here, the slots carry each solve's mask bits
+  Vec6 zero{};
+  run<kModeMatvec, RVec<S>>(RVec<S>{}, nullptr, nullptr, RVec<S>{}, zero, 1.0, zero, nullptr,
+                            nullptr, s, slots, nb);
   return zinv_blocks();
 }
 

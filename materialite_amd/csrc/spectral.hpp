@@ -17,6 +17,28 @@ __host__ __device__ constexpr int sym6(int i, int j) {
   return i <= j ? i * 6 - (i * (i - 1)) / 2 + (j - i) : j * 6 - (j * (j - 1)) / 2 + (i - j);
 }
 
+// Per-solve pointers of one MINRES solve in a batch of same-size solves (one
+// launch of each kernel serves every solve of the batch: blockIdx.y, or
+// blockIdx.x in the one-workgroup scalar kernels, selects the slot).  Roles are
+// those of one iteration: r1 / r2 the Lanczos vectors before the swap that
+// follows k_lanczos (the matvec reads r2 and updates r1; k_update reads r2 as
+// v), w1 / w2 the previous w vectors and wf the one written.
+template <typename S>
+struct BatchSlot {
+  CVec<S> spec;
+  const work_t<S>* tangent;
+  RVec<S> r1, r2;
+  int maskbits;
+  MinresState* st;
+  double* partials;
+  work_t<S>* w1;
+  work_t<S>* w2;
+  work_t<S>* wf;
+  work_t<S>* x;
+  const double* b;  // right-hand side ([6][n] FP64), read at the start
+  double rtol;
+};
+
 // S: storage of the spectrum and of the Lanczos vectors in HBM (float,
 // double or f48); the tangent and the MINRES w / x vectors are work_t<S>.
 // A: arithmetic type of the FFT passes (LDS tiles, twiddles, Ghat).
@@ -47,6 +69,11 @@ class Spectral {
   // Returns the number of partial sums written.
   int matvec_minres(RVec<S> r2, const W* tangent, RVec<S> r1, const Vec6& mask,
                     const MinresState* st, double* partials, hipStream_t s);
+  // The same matvec for nb solves of this grid at once, each from its slot
+  // (slots: device array of nb entries).  Returns the partial sums per solve.
+  int matvec_batch(const BatchSlot<S>* slots, int nb, hipStream_t s);
+  CVec<S> spectrum() const { return spec_; }
+  static int mask_bits(const Vec6& mask);
 
   int zinv_blocks() const;
 
@@ -54,7 +81,8 @@ class Spectral {
   void plan(int nx, int ny, int nz);
   template <int MODE, typename In>
   void run(In in, const W* tangent, double* out, RVec<S> r1, const Vec6& a, double sgn,
-           const Vec6& mask, const MinresState* st, double* partials, hipStream_t s);
+           const Vec6& mask, const MinresState* st, double* partials, hipStream_t s,
+           const BatchSlot<S>* slots = nullptr, int nb = 1);
 
   Grid g_;
   AxisPlan px_, py_, pz_;

@@ -130,131 +130,224 @@ class SmallStrainFFT:
         converged increment with a dict of the increment's counters; a true
         return value ends the run there.
         """
-        self._sizes = material.sizes
-        self._dimensions = material.dimensions
-        self._num_points = material.num_points
-        time = self.start_time
-        time_tolerance = self.min_time_increment / 2.0
-        if output_times is not None:
-            time_idx = 0
-            output_times = np.append(output_times, np.inf)
-            next_output_time = output_times[time_idx]
-        else:
-            next_output_time = np.inf
-        new_state = defaultdict(list)
-
-        orientations = material.extract(orientation_label)
-        rotations = np.broadcast_to(orientations.rotation_matrix,
-                                    (self._num_points, 3, 3))
-        models, model_of_point = self._get_constitutive_model(
-            material, phase_label, self._constitutive_model)
-
-        native = NativeSolver(self._dimensions, self.precision)
+        run = self.start(material, orientation_label=orientation_label, phase_label=phase_label,
+                         output_variables=output_variables, output_times=output_times,
+                         global_tolerance=global_tolerance,
+                         strain_correction_tolerance=strain_correction_tolerance,
+                         linear_solver_tolerance=linear_solver_tolerance,
+                         postprocessor=postprocessor, increment_callback=increment_callback)
         try:
-            for m in models:
-                self._register(native, m)
-            native.set_microstructure(rotations, model_of_point)
-            native.initialize()
-
-            ls = self.load_schedule
-            mask = np.asarray(ls.stress_mask, float)
-            old_max_strain_increment = 0.0
-            summed_von_mises_stress = 0.0
-            time_step_id = 0
-            time_increment = self.initial_time_increment
-            krylov = newton = 0
-            floored = False
-            # every attempted increment: (time, dt, converged, Newton iterations,
-            # Krylov iterations, constitutive failure); kept up to date so that
-            # it survives a "min time increment reached" error
-            history = []
-            self.stats = {"history": history}
-            while time < (self.end_time - time_tolerance):
-                self._logger.info(f"New increment: time {time} + {time_increment}")
-                d_strain = ls.strain_increment(time, time_increment).components
-                d_stress = ls.stress_increment(time, time_increment).components
-                applied = ls.stress(time, time_increment).components
-                max_strain_increment = d_strain[np.argmax(np.abs(d_strain))]
-                p = StepParams()
-                p.strain_increment[:] = [float(v) for v in d_strain]
-                p.stress_increment[:] = [float(v) for v in d_stress]
-                p.applied_stress[:] = [float(v) for v in applied]
-                p.stress_mask[:] = [float(v) for v in mask]
-                p.time_increment = float(time_increment)
-                p.global_tolerance = float(global_tolerance)
-                p.strain_correction_tolerance = float(strain_correction_tolerance)
-                p.linear_solver_tolerance = float(linear_solver_tolerance)
-                p.summed_von_mises_stress = float(summed_von_mises_stress)
-                p.time_step_id = int(time_step_id)
-                p.reset_fluctuation = int(max_strain_increment * old_max_strain_increment < 0.0)
-                rep = native.step(p)
-                if rep.linear_solver_tolerance_used > linear_solver_tolerance and not floored:
-                    floored = True
-                    self._logger.warning(
-                        f"precision {self.precision!r}: linear_solver_tolerance "
-                        f"{linear_solver_tolerance:g} raised to {rep.linear_solver_tolerance_used:g}")
-                krylov += rep.krylov_iterations
-                newton += rep.newton_iterations
-                history.append((float(time), float(time_increment), bool(rep.converged),
-                                int(rep.newton_iterations), int(rep.krylov_iterations),
-                                bool(rep.constitutive_failed)))
-                self._logger.debug(f"global iterations {rep.newton_iterations}, "
-                                   f"linear solver iterations {rep.krylov_iterations}, "
-                                   f"equilibrium error {rep.equilibrium_error}")
-                if not rep.converged and not rep.constitutive_failed:
-                    self._logger.info("Too many iterations in macroloading loop")
-                if rep.converged:
-                    summed_von_mises_stress += rep.mean_von_mises_stress
-                    old_max_strain_increment = max_strain_increment
-                    time_step_id += 1
-                    time += time_increment
-                    # the reference collects outputs, advances the output time and then
-                    # sizes the next increment (small_strain_fft.py:221-236); the device
-                    # commit (state, and fluctuation scaled by the new increment) comes
-                    # first here, which leaves the collected fields unchanged
-                    output_now = time >= (next_output_time - time_tolerance)
-                    upcoming = output_times[time_idx + 1] if output_now else next_output_time
-                    new_time_increment = self._new_time_increment(
-                        time, time_increment, upcoming, rep.max_constitutive_iterations)
-                    native.commit(new_time_increment / time_increment)
-                    if output_now:
-                        outputs = self._outputs(native, models, output_variables)
-                        outputs.update({"stress": Order2SymmetricTensor(native.get_field("stress"), "p"),
-                                        "strain": Order2SymmetricTensor(native.get_field("strain"), "p")})
-                        if postprocessor is not None:
-                            outputs = postprocessor(outputs)
-                        for k, v in outputs.items():
-                            new_state[k].append(v)
-                        time_idx += 1
-                        next_output_time = output_times[time_idx]
-                    time_increment = new_time_increment
-                    if increment_callback is not None and increment_callback({
-                            "step": time_step_id, "time": time,
-                            "krylov_iterations": int(rep.krylov_iterations),
-                            "newton_iterations": int(rep.newton_iterations),
-                            "constitutive_iterations": int(rep.max_constitutive_iterations),
-                            "num_points": self._num_points}):
-                        break
-                else:
-                    time_increment = time_increment / 2.0
-                    native.rollback()
-                    if time_increment < self.min_time_increment:
-                        raise ValueError("min time increment reached")
-
-            outputs = self._outputs(native, models, output_variables)
-            outputs.update({"stress": Order2SymmetricTensor(native.get_field("stress"), "p"),
-                            "strain": Order2SymmetricTensor(native.get_field("strain"), "p")})
-            self.stats = {"krylov_iterations": int(krylov), "newton_iterations": int(newton),
-                          "steps": int(time_step_id), "history": history,
-                          "linear_solver_tolerance_used": max(
-                              float(linear_solver_tolerance),
-                              1.0e-5 if self.precision == "mixed" else 0.0)}
+            while not run.finished:
+                run.apply(run.native.step(run.next_params()))
+            return run.result()
         finally:
-            native.close()
+            run.close()
 
-        if output_times is None:
+    def start(self, material, **kwargs):
+        """The run of ``material`` (the keywords of ``run``) as an object that
+        is advanced one load increment at a time: ``next_params()`` gives the
+        increment's device parameters and ``apply(report)`` takes the device's
+        report of it.  ``run`` drives one such object; ``run_batch`` drives
+        several in lockstep so that their MINRES solves share launches."""
+        return _Run(self, material, **kwargs)
+
+    @staticmethod
+    def run_batch(jobs):
+        """Solves several (solver, material, keywords) jobs of one grid size and
+        precision together: every round, the unfinished runs each take one load
+        increment on their own schedule and time step, and the MINRES solves of
+        those increments run as one batch on the device (mtl_solver_step_batch).
+        Returns the materials in the order of the jobs; each is the result
+        ``solver.run(material, **keywords)`` gives."""
+        runs = [solver.start(material, **kw) for solver, material, kw in jobs]
+        try:
+            drive_batch(runs)
+            return [r.result() for r in runs]
+        finally:
+            for r in runs:
+                r.close()
+
+
+def drive_batch(runs):
+    """Advances the given runs (same grid and precision) until all have
+    finished, one increment of every unfinished run per device call."""
+    while True:
+        live = [r for r in runs if not r.finished]
+        if not live:
+            return
+        reps = NativeSolver.step_batch([r.native for r in live], [r.next_params() for r in live])
+        for r, rep in zip(live, reps):
+            r.apply(rep)
+
+
+class _Run:
+    """The time loop of one SmallStrainFFT.run (small_strain_fft.py:106-246):
+    adaptive increment, cutback, output times and post-processor, with each
+    increment's predictor, Newton loop and MINRES corrections on the device."""
+
+    def __init__(self, solver, material, orientation_label="orientation", phase_label=None,
+                 output_variables=None, output_times=None, global_tolerance=1.0e-3,
+                 strain_correction_tolerance=1.0e-2, linear_solver_tolerance=1.0e-5,
+                 postprocessor=None, increment_callback=None):
+        sv = self.solver = solver
+        sv._sizes = material.sizes
+        sv._dimensions = material.dimensions
+        sv._num_points = material.num_points
+        self.material = material
+        self.output_variables = output_variables
+        self.postprocessor = postprocessor
+        self.increment_callback = increment_callback
+        self.global_tolerance = global_tolerance
+        self.strain_correction_tolerance = strain_correction_tolerance
+        self.linear_solver_tolerance = linear_solver_tolerance
+        self.time = sv.start_time
+        self.time_tolerance = sv.min_time_increment / 2.0
+        self.output_times = None
+        if output_times is not None:
+            self.time_idx = 0
+            self.output_times = np.append(output_times, np.inf)
+            self.next_output_time = self.output_times[0]
+        else:
+            self.next_output_time = np.inf
+        self.new_state = defaultdict(list)
+        orientations = material.extract(orientation_label)
+        rotations = np.broadcast_to(orientations.rotation_matrix, (sv._num_points, 3, 3))
+        self.models, model_of_point = sv._get_constitutive_model(
+            material, phase_label, sv._constitutive_model)
+        self.native = NativeSolver(sv._dimensions, sv.precision)
+        try:
+            for m in self.models:
+                sv._register(self.native, m)
+            self.native.set_microstructure(rotations, model_of_point)
+            self.native.initialize()
+        except BaseException:
+            self.native.close()
+            raise
+        self.mask = np.asarray(sv.load_schedule.stress_mask, float)
+        self.old_max_strain_increment = 0.0
+        self.summed_von_mises_stress = 0.0
+        self.time_step_id = 0
+        self.time_increment = sv.initial_time_increment
+        self.krylov = self.newton = 0
+        self.floored = False
+        self.stopped = False
+        self.final = None
+        # every attempted increment: (time, dt, converged, Newton iterations,
+        # Krylov iterations, constitutive failure); kept up to date so that it
+        # survives a "min time increment reached" error
+        self.history = []
+        sv.stats = {"history": self.history}
+
+    @property
+    def finished(self):
+        return self.stopped or self.time >= (self.solver.end_time - self.time_tolerance)
+
+    def next_params(self):
+        sv, ls = self.solver, self.solver.load_schedule
+        t, dt = self.time, self.time_increment
+        sv._logger.info(f"New increment: time {t} + {dt}")
+        d_strain = ls.strain_increment(t, dt).components
+        d_stress = ls.stress_increment(t, dt).components
+        applied = ls.stress(t, dt).components
+        self._max_strain_increment = d_strain[np.argmax(np.abs(d_strain))]
+        p = StepParams()
+        p.strain_increment[:] = [float(v) for v in d_strain]
+        p.stress_increment[:] = [float(v) for v in d_stress]
+        p.applied_stress[:] = [float(v) for v in applied]
+        p.stress_mask[:] = [float(v) for v in self.mask]
+        p.time_increment = float(dt)
+        p.global_tolerance = float(self.global_tolerance)
+        p.strain_correction_tolerance = float(self.strain_correction_tolerance)
+        p.linear_solver_tolerance = float(self.linear_solver_tolerance)
+        p.summed_von_mises_stress = float(self.summed_von_mises_stress)
+        p.time_step_id = int(self.time_step_id)
+        p.reset_fluctuation = int(self._max_strain_increment * self.old_max_strain_increment < 0.0)
+        return p
+
+    def apply(self, rep):
+        sv, native = self.solver, self.native
+        if rep.linear_solver_tolerance_used > self.linear_solver_tolerance and not self.floored:
+            self.floored = True
+            sv._logger.warning(
+                f"precision {sv.precision!r}: linear_solver_tolerance "
+                f"{self.linear_solver_tolerance:g} raised to {rep.linear_solver_tolerance_used:g}")
+        self.krylov += rep.krylov_iterations
+        self.newton += rep.newton_iterations
+        self.history.append((float(self.time), float(self.time_increment), bool(rep.converged),
+                             int(rep.newton_iterations), int(rep.krylov_iterations),
+                             bool(rep.constitutive_failed)))
+        sv._logger.debug(f"global iterations {rep.newton_iterations}, "
+                         f"linear solver iterations {rep.krylov_iterations}, "
+                         f"equilibrium error {rep.equilibrium_error}")
+        if not rep.converged and not rep.constitutive_failed:
+            sv._logger.info("Too many iterations in macroloading loop")
+        if not rep.converged:
+            self.time_increment = self.time_increment / 2.0
+            native.rollback()
+            if self.time_increment < sv.min_time_increment:
+                raise ValueError("min time increment reached")
+            return
+        self.summed_von_mises_stress += rep.mean_von_mises_stress
+        self.old_max_strain_increment = self._max_strain_increment
+        self.time_step_id += 1
+        self.time += self.time_increment
+        # the reference collects outputs, advances the output time and then
+        # sizes the next increment (small_strain_fft.py:221-236); the device
+        # commit (state, and fluctuation scaled by the new increment) comes
+        # first here, which leaves the collected fields unchanged
+        output_now = self.time >= (self.next_output_time - self.time_tolerance)
+        upcoming = self.output_times[self.time_idx + 1] if output_now else self.next_output_time
+        new_time_increment = sv._new_time_increment(
+            self.time, self.time_increment, upcoming, rep.max_constitutive_iterations)
+        native.commit(new_time_increment / self.time_increment)
+        if output_now:
+            outputs = self._fields()
+            if self.postprocessor is not None:
+                outputs = self.postprocessor(outputs)
+            for k, v in outputs.items():
+                self.new_state[k].append(v)
+            self.time_idx += 1
+            self.next_output_time = self.output_times[self.time_idx]
+        self.time_increment = new_time_increment
+        if self.increment_callback is not None and self.increment_callback({
+                "step": self.time_step_id, "time": self.time,
+                "krylov_iterations": int(rep.krylov_iterations),
+                "newton_iterations": int(rep.newton_iterations),
+                "constitutive_iterations": int(rep.max_constitutive_iterations),
+                "num_points": sv._num_points}):
+            self.stopped = True
+        if self.finished:
+            self._finish()
+
+    def _fields(self):
+        native = self.native
+        outputs = self.solver._outputs(native, self.models, self.output_variables)
+        outputs.update({"stress": Order2SymmetricTensor(native.get_field("stress"), "p"),
+                        "strain": Order2SymmetricTensor(native.get_field("strain"), "p")})
+        return outputs
+
+    def _finish(self):
+        sv = self.solver
+        self.final = self._fields()
+        sv.stats = {"krylov_iterations": int(self.krylov), "newton_iterations": int(self.newton),
+                    "steps": int(self.time_step_id), "history": self.history,
+                    "linear_solver_tolerance_used": max(
+                        float(self.linear_solver_tolerance),
+                        1.0e-5 if sv.precision == "mixed" else 0.0)}
+        self.close()
+
+    def close(self):
+        if self.native is not None:
+            self.native.close()
+
+    def result(self):
+        """The new Material (small_strain_fft.py:248-264)."""
+        if self.final is None:
+            self._finish()
+        material, outputs, new_state = self.material, self.final, self.new_state
+        if self.output_times is None:
             return material.create_fields(outputs)
-        if postprocessor is None:
+        if self.postprocessor is None:
             for k, v in new_state.items():
                 new_state[k] = type(v[0]).from_stack(v, new_dim="t", axis=len(v[0].dims_str))
             return material.create_fields(dict(new_state))

@@ -3,20 +3,24 @@
 A ``RveServer`` takes independent ``SmallStrainFFT`` solves as they arrive and
 runs them with continuous batching:
 
-* every admitted request owns a solver handle -- its own HIP stream, and its
-  fields, tangents, Krylov vectors and hardening state resident in HBM for all
-  of its load steps;
+* every admitted request owns a solver handle: its fields, tangents, Krylov
+  vectors and hardening state stay resident in HBM for all of its load steps;
 * a request is admitted (FIFO) as soon as a slot is free and its HBM footprint
   fits the budget left by the requests in flight; requests that arrive while
   others run join them without waiting for the batch to drain;
-* each admitted request runs in a worker thread; the library's calls release
-  the GIL (ctypes), so the device work of all requests in flight overlaps on
-  the GPU, each on its own stream.
+* requests of the same grid size and precision (``Request.job`` set) share a
+  lane: one worker thread advances all of them one load increment per device
+  call (mtl_solver_step_batch), so the MINRES iterations of every solve in the
+  lane run as one launch of each kernel; a request joins its lane at the next
+  increment and leaves it when its own schedule ends;
+* other requests run in worker threads of their own; the library's calls
+  release the GIL (ctypes), so the device work of all lanes and threads
+  overlaps on the GPU.
 
-A solve's arithmetic does not depend on what else is in flight (no shared
-buffers, no data-dependent scheduling across handles), so a result is the same
-alone and in a mix.  Per-request times (arrival, admission, completion) give
-the latency distribution reported beside throughput.
+A solve's arithmetic does not depend on what else is in flight (per-solve
+buffers, scalars and stopping tests, also inside a batch), so a result is the
+same alone and in a mix.  Per-request times (arrival, admission, completion)
+give the latency distribution reported beside throughput.
 
 The reference has no serving layer; this module is the request loop that the
 mixed-size stream workload runs on.
@@ -70,10 +74,15 @@ def footprint_bytes(dims, precision="mixed", evp_points=0, num_slip_systems=12):
 
 @dataclass
 class Request:
-    """One RVE solve: ``run()`` performs it (e.g. a bound ``SmallStrainFFT.run``)."""
+    """One RVE solve: ``run()`` performs it (e.g. a bound ``SmallStrainFFT.run``).
+    With ``job = (solver, material, run keywords)`` and ``finish`` (the
+    returned Material -> the request's result) the server may batch it with
+    other requests of its grid size and precision instead."""
     run: callable
     footprint: int
     tag: object = None
+    job: tuple = None
+    finish: callable = None
     arrival: float = None
     admitted: float = None
     finished: float = None
@@ -87,11 +96,15 @@ class Request:
 
 
 class RveServer:
-    """Continuous-batching request loop with admission by HBM footprint."""
+    """Continuous-batching request loop with admission by HBM footprint; with
+    ``batching`` same-size requests share device launches (see the module
+    docstring)."""
 
-    def __init__(self, hbm_budget, max_slots=8):
+    def __init__(self, hbm_budget, max_slots=8, batching=False):
         if max_slots < 1:
             raise ValueError("max_slots must be >= 1")
+        self.batching = bool(batching)
+        self._lanes = {}  # (dims, precision) -> lane waiting list
         self.hbm_budget = int(hbm_budget)
         self.max_slots = int(max_slots)
         self._queue = deque()
@@ -145,25 +158,100 @@ class RveServer:
                 self.peak_bytes = max(self.peak_bytes, self._bytes)
                 self.peak_in_flight = max(self.peak_in_flight, self._in_flight)
             req.admitted = time.perf_counter()
+            if self.batching and req.job is not None:
+                self._to_lane(req)
+                continue
             t = threading.Thread(target=self._work, args=(req,), daemon=True)
             self._threads.append(t)
             t.start()
 
-    def _work(self, req):
-        try:
-            req.result = req.run()
-        except BaseException as e:  # reported through the future
-            req.error = e
+    # --------------------------------------------------------------- lanes
+    @staticmethod
+    def _lane_key(req):
+        solver, material, _ = req.job
+        return (tuple(int(d) for d in material.dimensions), solver.precision)
+
+    def _to_lane(self, req):
+        key = self._lane_key(req)
+        with self._cv:
+            lane = self._lanes.get(key)
+            if lane is not None:  # the lane's worker picks it up at its next increment
+                lane.append(req)
+                return
+            self._lanes[key] = [req]
+        t = threading.Thread(target=self._lane_work, args=(key,), daemon=True)
+        self._threads.append(t)
+        t.start()
+
+    def _lane_work(self, key):
+        from .models.small_strain_fft.small_strain_fft import NativeSolver
+
+        live = []  # (request, run)
+        while True:
+            with self._cv:
+                joining = self._lanes[key]
+                self._lanes[key] = []
+                if not joining and not live:
+                    del self._lanes[key]
+                    return
+            for req in joining:
+                solver, material, kw = req.job
+                try:
+                    live.append((req, solver.start(material, **kw)))
+                except BaseException as e:
+                    self._done(req, error=e)
+            stepping = [(q, r) for q, r in live if not r.finished]
+            if stepping:
+                try:
+                    params = [r.next_params() for _, r in stepping]
+                    reps = NativeSolver.step_batch([r.native for _, r in stepping], params)
+                except BaseException as e:  # every run of the failed device call ends with it
+                    reps = None
+                    for _, r in stepping:
+                        r.stopped, r.error = True, e
+                for (q, r), rp in zip(stepping, reps or []):
+                    try:
+                        r.apply(rp)
+                    except BaseException as e:
+                        r.stopped, r.error = True, e
+            keep = []
+            for q, r in live:
+                if not r.finished:
+                    keep.append((q, r))
+                    continue
+                err = getattr(r, "error", None)
+                if err is not None:
+                    r.close()
+                    self._done(q, error=err)
+                    continue
+                try:
+                    self._done(q, result=q.finish(r.result()))
+                except BaseException as e:
+                    self._done(q, error=e)
+                finally:
+                    r.close()
+            live = keep
+
+    def _done(self, req, result=None, error=None):
+        req.result, req.error = result, error
         req.finished = time.perf_counter()
         with self._cv:
             self._in_flight -= 1
             self._bytes -= req.footprint
             self.completed.append(req)
             self._cv.notify_all()
-        if req.error is not None:
-            req.future.set_exception(req.error)
+        if error is not None:
+            req.future.set_exception(error)
         else:
-            req.future.set_result(req.result)
+            req.future.set_result(result)
+
+    def _work(self, req):
+        try:
+            result = req.run()
+        except BaseException as e:  # reported through the future
+            self._done(req, error=e)
+            return
+        self._done(req, result=result)
 
 
 def latency_summary(requests):
@@ -316,8 +404,7 @@ def build_stream_request(spec, precision="refined", keep_material=False):
                             initial_time_increment=spec.dt, max_time_increment=spec.dt,
                             constitutive_model=model, precision=precision)
 
-    def run():
-        out = solver.run(material)
+    def finish(out):
         stress = out.extract("stress").components
         res = {"mean_stress": stress.mean(axis=0), "krylov": solver.stats["krylov_iterations"],
                "steps": solver.stats["steps"]}
@@ -325,6 +412,9 @@ def build_stream_request(spec, precision="refined", keep_material=False):
             res["stress"] = stress
         return res
 
+    def run():
+        return finish(solver.run(material))
+
     evp_points = spec.n ** 3 if spec.model == "evp" else 0
     return Request(run=run, footprint=footprint_bytes([spec.n] * 3, precision, evp_points),
-                   tag=spec)
+                   tag=spec, job=(solver, material, {}), finish=finish)
