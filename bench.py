@@ -79,6 +79,8 @@ def parse():
                     help="Poisson arrival rate of the serving stream (requests/s)")
     ap.add_argument("--stream-slots", type=int, default=8,
                     help="solves in flight at once in the serving stream")
+    ap.add_argument("--batching", type=int, default=1,
+                    help="with --stream-only: same-size requests share device launches")
     ap.add_argument("--precision-study", type=int, default=1,
                     help="64^3 elastic, uniaxial x then xy shear: mixed vs FP64 stress error")
     ap.add_argument("--large", type=int, default=1,
@@ -108,23 +110,26 @@ def parse():
     return ap.parse_args()
 
 
-def serving_stream(nreq, rate, slots):
-    """Mixed-size request stream: Poisson arrivals at `rate`, request j drawn
-    from seed 20264000 + j (grid 32/48/64/96^3, 50-500 grains, elastic or EVP,
-    one of six load directions, 20 steps of 0.5 s), continuous batching with
-    admission by HBM footprint.  Materials are generated before the clock
-    starts; latency runs from arrival to the returned Material."""
+def serving_stream(nreq, rate, slots, batching=False, sizes=(32, 48, 64, 96), lanes=4):
+    """Request stream: Poisson arrivals at `rate`, request j drawn from seed
+    20264000 + j (grids from `sizes`, 50-500 grains, elastic or EVP, one of
+    six load directions, 20 steps of 0.5 s), continuous batching with
+    admission by HBM footprint; with `batching`, same-size requests share
+    device launches in up to `lanes` lanes per size.  Materials are generated before the clock
+    starts; latency runs from arrival to the returned result."""
     import numpy as np
     import torch
 
     from materialite_amd import serving
 
-    specs = [serving.draw_stream_spec(20264000 + j) for j in range(nreq)]
-    print(f"[bench] stream: building {nreq} requests", file=sys.stderr, flush=True)
+    specs = [serving.draw_stream_spec(20264000 + j, sizes=sizes) for j in range(nreq)]
+    print(f"[bench] stream ({'batched' if batching else 'unbatched'}, {rate}/s, grids {sizes}): "
+          f"building {nreq} requests", file=sys.stderr, flush=True)
     reqs = [serving.build_stream_request(sp) for sp in specs]
     gaps = np.random.default_rng(20264999).exponential(1.0 / rate, nreq)
     budget = int(0.5 * torch.cuda.mem_get_info()[0])
-    server = serving.RveServer(hbm_budget=budget, max_slots=slots)
+    server = serving.RveServer(hbm_budget=budget, max_slots=slots, batching=batching,
+                               lanes_per_size=lanes)
     t0 = time.perf_counter()
     arrivals = t0 + np.cumsum(gaps) - gaps[0]
     futs = []
@@ -138,7 +143,7 @@ def serving_stream(nreq, rate, slots):
     for f in futs:
         f.result()
         done += 1
-        if done % 4 == 0:
+        if done % 8 == 0:
             print(f"[bench] stream: {done}/{nreq} solves done", file=sys.stderr, flush=True)
     server.close()
     summ = serving.latency_summary(reqs)
@@ -146,22 +151,34 @@ def serving_stream(nreq, rate, slots):
     span = max(r.finished for r in reqs) - min(r.arrival for r in reqs)
     summ.update({
         "arrival_rate_per_s": rate,
+        "sustained": summ["solves_per_s"] >= 0.95 * rate,
+        "batching": batching,
+        "lanes_per_size": lanes if batching else None,
         "slots": slots,
         "hbm_budget_gb": budget / 1e9,
         "peak_admitted_gb": server.peak_bytes / 1e9,
         "peak_in_flight": server.peak_in_flight,
         "voxel_iters_per_s": vi / span,
         "mix": {"evp": sum(sp.model == "evp" for sp in specs),
-                "grids": {str(n): sum(sp.n == n for sp in specs) for n in (32, 48, 64, 96)}},
+                "grids": {str(n): sum(sp.n == n for sp in specs) for n in sizes}},
         "precision": "refined",
         "seeds": "request j: 20264000 + j; arrivals: 20264999",
     })
     return summ
 
 
-# bytes per stored real value: tangent and MINRES w / x vectors, Lanczos
-# vectors, spectrum (refined: an FP32 value plus a bf16 remainder)
-WIDTHS = {"fp64": (8, 8, 8), "mixed": (4, 4, 4), "refined": (4, 6, 6)}
+def serving_report(nreq, slots):
+    """The mixed-size stream (32-96^3) batched and unbatched at a rate the
+    server sustains and at one above it, and a stream of 32^3-48^3 requests
+    (launch-bound sizes) batched and unbatched at a high rate."""
+    out = {"mixed_sizes": {}, "small_32_48": {}}
+    for label, rate in (("sustained_rate", 4.0), ("overload_rate", 12.0)):
+        out["mixed_sizes"][label] = {
+            "batched": serving_stream(nreq, rate, slots, batching=True),
+            "unbatched": serving_stream(nreq, rate, slots, batching=False)}
+    for label, batching in (("batched", True), ("unbatched", False)):
+        out["small_32_48"][label] = serving_stream(nreq, 40.0, 16, batching=batching, sizes=(32, 48))
+    return out
 
 
 def bytes_per_vi(nz, precision):
@@ -785,7 +802,7 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.stream_only:
         print(json.dumps({"serving": serving_stream(args.stream_requests, args.stream_rate,
-                                                    args.stream_slots)}))
+                                                    args.stream_slots, batching=args.batching)}))
         return
     if args.copy_probe:
         src = torch.ones(args.copy_probe * 2 ** 18, dtype=torch.float32, device="cuda")
@@ -910,7 +927,7 @@ def main():
         full = {"increments": fr["steps"], "krylov_iterations": fr["krylov"],
                 "value": npts * fr["krylov"] / fr["seconds"],
                 "ms_per_step": 1e3 * fr["seconds"] / max(fr["steps"], 1)}
-    stream = (serving_stream(args.stream_requests, args.stream_rate, args.stream_slots)
+    stream = (serving_report(args.stream_requests, args.stream_slots)
               if args.stream_requests > 0 else None)
     study = precision_study() if args.precision_study else None
     large = large_two_phase(converged=bool(args.large_converged)) if args.large else None

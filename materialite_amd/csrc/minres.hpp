@@ -1,6 +1,8 @@
 // Device MINRES for the Galerkin system  Ghat * (D : x) = b.
 #pragma once
 
+#include <array>
+#include <map>
 #include <vector>
 
 #include "spectral.hpp"
@@ -38,7 +40,10 @@ class Minres {
   // (x0 = 0 each): every iteration is one launch of each kernel for all of
   // them, with per-system scalars, stopping tests and iteration counts
   // (itns[i]); a system that has stopped costs its workgroups one flag read.
-  // Each result is the one solve() gives for the same system.
+  // Each result is the one solve() gives for the same system.  With
+  // MTL_BATCH_GRAPHS=1 (and kernel timing off) each iteration is the replay
+  // of a recorded graph of its ten launches, kept by the first solver of the
+  // batch per batch size.
   static void solve_batch(const std::vector<BatchItem>& items, long long maxiter, hipStream_t s,
                           int* itns);
   // Host copy of the last state (valid after solve()).
@@ -59,6 +64,10 @@ class Minres {
   BatchSlot<S>* slots_dev_ = nullptr;
   BatchSlot<S>* slots_host_ = nullptr;
   int slot_cap_ = 0;
+  // one-iteration graphs of solve_batch by batch size, one per role phase;
+  // they hold the slot table's address
+  std::map<int, std::array<hipGraphExec_t, 6>> graphs_;
+  void drop_graphs();
 };
 
 }  // namespace mtl

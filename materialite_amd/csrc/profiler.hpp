@@ -13,6 +13,8 @@
 
 #include <atomic>
 #include <mutex>
+#include <tuple>
+#include <utility>
 #include <vector>
 
 #include "common.hpp"
@@ -45,6 +47,67 @@ enum KernelId {
 // matvec pass id -> the residual-projection id of the same pass
 constexpr int residual_id(int matvec_id) { return matvec_id + kKResidualZForward; }
 
+// Records kernel launches as the nodes of a graph instead of launching them:
+// while a recorder is active on a thread, every Profiler::launch made on that
+// thread appends a kernel node that depends on the previous one, so the graph
+// replays the launches in order on the stream it is launched on.  Nothing is
+// captured from a stream, so other threads' stream work is unaffected.
+class GraphRecorder {
+ public:
+  GraphRecorder() {
+    MTL_HIP(hipGraphCreate(&graph_, 0));
+    prev_ = active();
+    active() = this;
+  }
+  ~GraphRecorder() {
+    active() = prev_;
+    if (graph_) (void)hipGraphDestroy(graph_);
+  }
+  GraphRecorder(const GraphRecorder&) = delete;
+  GraphRecorder& operator=(const GraphRecorder&) = delete;
+
+  static GraphRecorder*& active() {
+    static thread_local GraphRecorder* r = nullptr;
+    return r;
+  }
+
+  template <typename... KA, typename... Args>
+  void add(void (*kernel)(KA...), dim3 grid, dim3 block, size_t lds, Args... args) {
+    static_assert(sizeof...(KA) == sizeof...(Args), "kernel argument count");
+    std::tuple<KA...> vals(static_cast<KA>(args)...);
+    add_node(reinterpret_cast<const void*>(kernel), grid, block, lds, vals,
+             std::index_sequence_for<KA...>{});
+  }
+
+  // the recorded launches as an executable graph (ends the recording)
+  hipGraphExec_t instantiate() {
+    active() = prev_;
+    hipGraphExec_t exec = nullptr;
+    MTL_HIP(hipGraphInstantiate(&exec, graph_, nullptr, nullptr, 0));
+    return exec;
+  }
+
+ private:
+  template <typename Tup, size_t... I>
+  void add_node(const void* fn, dim3 grid, dim3 block, size_t lds, Tup& vals,
+                std::index_sequence<I...>) {
+    void* params[sizeof...(I) + 1] = {static_cast<void*>(&std::get<I>(vals))...};
+    hipKernelNodeParams p{};
+    p.func = const_cast<void*>(fn);
+    p.gridDim = grid;
+    p.blockDim = block;
+    p.sharedMemBytes = unsigned(lds);
+    p.kernelParams = params;
+    p.extra = nullptr;
+    hipGraphNode_t node = nullptr;
+    MTL_HIP(hipGraphAddKernelNode(&node, graph_, last_ ? &last_ : nullptr, last_ ? 1 : 0, &p));
+    last_ = node;
+  }
+  hipGraph_t graph_ = nullptr;
+  hipGraphNode_t last_ = nullptr;
+  GraphRecorder* prev_ = nullptr;
+};
+
 // Handles used from several threads at once (the serving layer) share the
 // profiler: with timing off a launch only reads an atomic flag; with timing on the
 // records are kept under a mutex.
@@ -52,6 +115,10 @@ class Profiler {
  public:
   template <typename F, typename... Args>
   void launch(int id, F kernel, dim3 grid, dim3 block, size_t lds, hipStream_t s, Args... args) {
+    if (GraphRecorder* r = GraphRecorder::active()) {
+      r->add(kernel, grid, block, lds, args...);
+      return;
+    }
     if (!enabled_.load(std::memory_order_relaxed)) {
       hipExtLaunchKernelGGL(kernel, grid, block, std::uint32_t(lds), s, nullptr, nullptr, 0u,
                             args...);
@@ -74,6 +141,8 @@ class Profiler {
     std::lock_guard<std::mutex> lock(mu_);
     wake_next_ = true;
   }
+
+  bool enabled() const { return enabled_.load(std::memory_order_relaxed); }
 
   void set_enabled(bool on) {
     std::lock_guard<std::mutex> lock(mu_);

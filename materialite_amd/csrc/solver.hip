@@ -10,8 +10,10 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <exception>
 #include <memory>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "../../include/materialite_amd.h"
@@ -343,6 +345,40 @@ class ModelSet {
   std::vector<std::unique_ptr<Model<K>>> models_;
 };
 
+// f(i) for every i of idx, on one host thread each (the caller's thread
+// takes the first), with the caller's device current; the first exception
+// raised is rethrown once all have ended.
+template <typename F>
+void for_each_parallel(const std::vector<int>& idx, F f) {
+  const size_t n = idx.size();
+  if (n <= 1) {
+    for (int i : idx) f(i);
+    return;
+  }
+  int dev = 0;
+  MTL_HIP(hipGetDevice(&dev));
+  std::vector<std::exception_ptr> err(n);
+  std::vector<std::thread> th;
+  th.reserve(n - 1);
+  for (size_t k = 1; k < n; ++k)
+    th.emplace_back([&, k] {
+      try {
+        MTL_HIP(hipSetDevice(dev));
+        f(idx[k]);
+      } catch (...) {
+        err[k] = std::current_exception();
+      }
+    });
+  try {
+    f(idx[0]);
+  } catch (...) {
+    err[0] = std::current_exception();
+  }
+  for (auto& t : th) t.join();
+  for (auto& e : err)
+    if (e) std::rethrow_exception(e);
+}
+
 // full Mandel 6x6 per point (N x 36) of a tangent [n/64][21][64]
 template <typename K>
 void tangent_to_host(const K* D, long long n, double* out) {
@@ -497,32 +533,49 @@ class Solver final : public SolverBase {
   }
 
   // The same increment for several solvers of one grid and form at once.
+  // The per-solver phases (constitutive update, residual, norms: each ends
+  // with a host read of a few reductions) run on one host thread per solver,
+  // each on its own stream, so that their launches and synchronisations
+  // overlap; only the MINRES solves are merged into shared launches.
   static void step_batch(const std::vector<Solver*>& hs, const mtl_step_params* P,
                          mtl_step_report* R) {
     const int nb = int(hs.size());
-    for (int i = 0; i < nb; ++i) hs[i]->step_begin(P[i], R[i]);
+    std::vector<int> all(nb);
+    for (int i = 0; i < nb; ++i) all[i] = i;
+    for_each_parallel(all, [&](int i) { hs[i]->step_begin(P[i], R[i]); });
     std::vector<typename Minres<S, A>::BatchItem> items;
-    std::vector<int> who, itns;
+    std::vector<int> live, who, itns;
+    std::vector<char> solve(nb);
     while (true) {
+      live.clear();
+      bool first = false;  // only a first Newton iteration has work before MINRES
+      for (int i = 0; i < nb; ++i)
+        if (!hs[i]->ss_.finished) {
+          live.push_back(i);
+          first = first || hs[i]->ss_.iteration == 0;
+        }
+      if (live.empty()) break;
+      auto pre = [&](int i) { solve[i] = hs[i]->step_pre_minres(R[i]) ? 1 : 0; };
+      if (first)
+        for_each_parallel(live, pre);
+      else
+        for (int i : live) pre(i);
       items.clear();
       who.clear();
-      bool any = false;
-      for (int i = 0; i < nb; ++i) {
-        Solver& h = *hs[i];
-        if (h.ss_.finished) continue;
-        any = true;
-        if (h.step_pre_minres(R[i])) {
+      for (int i : live)
+        if (solve[i]) {
+          Solver& h = *hs[i];
           items.push_back({h.minres_.get(), h.b_, h.D_, h.ss_.mask, h.ss_.lin_tol});
           who.push_back(i);
         }
-      }
-      if (!any) break;
       if (items.empty()) continue;
-      itns.assign(items.size(), 0);
+      itns.assign(nb, 0);
+      std::vector<int> got(items.size(), 0);
       // every solver's stream has drained (its residual norms were read on
       // the host); the batch runs on the first solver's stream
-      Minres<S, A>::solve_batch(items, 5LL * 6LL * hs[0]->n_, hs[who[0]]->s_, itns.data());
-      for (size_t k = 0; k < who.size(); ++k) hs[who[k]]->step_post_minres(itns[k], R[who[k]]);
+      Minres<S, A>::solve_batch(items, 5LL * 6LL * hs[0]->n_, hs[who[0]]->s_, got.data());
+      for (size_t k = 0; k < who.size(); ++k) itns[who[k]] = got[k];
+      for_each_parallel(who, [&](int i) { hs[i]->step_post_minres(itns[i], R[i]); });
     }
     for (int i = 0; i < nb; ++i) hs[i]->step_end(R[i]);
   }

@@ -8,11 +8,16 @@ runs them with continuous batching:
 * a request is admitted (FIFO) as soon as a slot is free and its HBM footprint
   fits the budget left by the requests in flight; requests that arrive while
   others run join them without waiting for the batch to drain;
-* requests of the same grid size and precision (``Request.job`` set) share a
-  lane: one worker thread advances all of them one load increment per device
-  call (mtl_solver_step_batch), so the MINRES iterations of every solve in the
-  lane run as one launch of each kernel; a request joins its lane at the next
-  increment and leaves it when its own schedule ends;
+* with ``batching``, requests of the same grid size and precision
+  (``Request.job`` set) share lanes: one worker thread per lane advances all
+  of its requests one load increment per device call
+  (mtl_solver_step_batch), so the MINRES iterations of every solve in the
+  lane run as one launch of each kernel; a request joins a lane at the next
+  increment and leaves it when its own schedule ends.  Up to
+  ``lanes_per_size`` lanes of one size run side by side (each on its own
+  streams: the device takes kernels from several hardware queues at once),
+  and a new request goes to a new lane while that cap allows, else to the
+  least loaded one;
 * other requests run in worker threads of their own; the library's calls
   release the GIL (ctypes), so the device work of all lanes and threads
   overlaps on the GPU.
@@ -95,16 +100,28 @@ class Request:
         return None if self.finished is None else self.finished - self.arrival
 
 
+class _Lane:
+    """Same-size requests advanced together by one worker thread."""
+
+    def __init__(self, key):
+        self.key = key
+        self.joining = []  # admitted, not yet started
+        self.load = 0      # requests in the lane (joining or running)
+
+
 class RveServer:
     """Continuous-batching request loop with admission by HBM footprint; with
     ``batching`` same-size requests share device launches (see the module
     docstring)."""
 
-    def __init__(self, hbm_budget, max_slots=8, batching=False):
+    def __init__(self, hbm_budget, max_slots=8, batching=False, lanes_per_size=4):
         if max_slots < 1:
             raise ValueError("max_slots must be >= 1")
+        if lanes_per_size < 1:
+            raise ValueError("lanes_per_size must be >= 1")
         self.batching = bool(batching)
-        self._lanes = {}  # (dims, precision) -> lane waiting list
+        self.lanes_per_size = int(lanes_per_size)
+        self._lanes = {}  # (dims, precision) -> running _Lane objects
         self.hbm_budget = int(hbm_budget)
         self.max_slots = int(max_slots)
         self._queue = deque()
@@ -174,32 +191,36 @@ class RveServer:
     def _to_lane(self, req):
         key = self._lane_key(req)
         with self._cv:
-            lane = self._lanes.get(key)
-            if lane is not None:  # the lane's worker picks it up at its next increment
-                lane.append(req)
+            lanes = self._lanes.setdefault(key, [])
+            lane = min(lanes, key=lambda ln: ln.load, default=None)
+            if lane is not None and (lane.load == 0 or len(lanes) >= self.lanes_per_size):
+                lane.joining.append(req)  # picked up at the lane's next increment
+                lane.load += 1
                 return
-            self._lanes[key] = [req]
-        t = threading.Thread(target=self._lane_work, args=(key,), daemon=True)
+            lane = _Lane(key)
+            lane.joining.append(req)
+            lane.load = 1
+            lanes.append(lane)
+        t = threading.Thread(target=self._lane_work, args=(lane,), daemon=True)
         self._threads.append(t)
         t.start()
 
-    def _lane_work(self, key):
+    def _lane_work(self, lane):
         from .models.small_strain_fft.small_strain_fft import NativeSolver
 
         live = []  # (request, run)
         while True:
             with self._cv:
-                joining = self._lanes[key]
-                self._lanes[key] = []
+                joining, lane.joining = lane.joining, []
                 if not joining and not live:
-                    del self._lanes[key]
+                    self._lanes[lane.key].remove(lane)
                     return
             for req in joining:
                 solver, material, kw = req.job
                 try:
                     live.append((req, solver.start(material, **kw)))
                 except BaseException as e:
-                    self._done(req, error=e)
+                    self._lane_done(lane, req, error=e)
             stepping = [(q, r) for q, r in live if not r.finished]
             if stepping:
                 try:
@@ -222,15 +243,20 @@ class RveServer:
                 err = getattr(r, "error", None)
                 if err is not None:
                     r.close()
-                    self._done(q, error=err)
+                    self._lane_done(lane, q, error=err)
                     continue
                 try:
-                    self._done(q, result=q.finish(r.result()))
+                    self._lane_done(lane, q, result=q.finish(r.result()))
                 except BaseException as e:
-                    self._done(q, error=e)
+                    self._lane_done(lane, q, error=e)
                 finally:
                     r.close()
             live = keep
+
+    def _lane_done(self, lane, req, result=None, error=None):
+        with self._cv:
+            lane.load -= 1
+        self._done(req, result=result, error=error)
 
     def _done(self, req, result=None, error=None):
         req.result, req.error = result, error

@@ -39,11 +39,15 @@ def _jobs(precision, n=24):
     return out
 
 
+@pytest.mark.parametrize("graphs", ["1", "0"])
 @pytest.mark.parametrize("precision", ["fp64", "refined", "mixed"])
-def test_batched_runs_match_single_runs(precision):
+def test_batched_runs_match_single_runs(precision, graphs, monkeypatch):
+    """With the batch iterations launched one by one (the default) and
+    replayed from recorded graphs."""
     from materialite_amd.models.small_strain_fft import SmallStrainFFT
 
-    print(f"batch {precision}", flush=True)
+    monkeypatch.setenv("MTL_BATCH_GRAPHS", graphs)
+    print(f"batch {precision} graphs {graphs}", flush=True)
     alone = []
     for solver, material, kw in _jobs(precision):
         out = solver.run(material, **kw)
@@ -62,16 +66,19 @@ def test_batched_runs_match_single_runs(precision):
     assert np.abs(pa - pb).max() <= 1e-12 * np.abs(pa).max()
 
 
-def test_serving_lane_matches_alone():
-    """A batching server: same-size requests share a lane and join it as they
-    arrive; every result equals the request's solve alone."""
+@pytest.mark.parametrize("lanes", [1, 2])
+def test_serving_lane_matches_alone(lanes):
+    """A batching server: same-size requests share lanes (one, or two side by
+    side) and join them as they arrive; every result equals the request's
+    solve alone."""
     specs = [serving.StreamSpec(seed=41 + k, n=32, grains=20 + 10 * k,
                                 model="evp" if k % 2 == 0 else "elastic",
                                 direction=serving.DIRECTIONS[k], steps=4) for k in range(4)]
     specs.append(serving.StreamSpec(seed=50, n=48, grains=30, model="evp", direction="zz", steps=3))
     alone = [serving.build_stream_request(sp, keep_material=True).run() for sp in specs]
+    print(f"serving lanes {lanes}", flush=True)
     server = serving.RveServer(hbm_budget=torch.cuda.mem_get_info()[0] // 2, max_slots=8,
-                               batching=True)
+                               batching=True, lanes_per_size=lanes)
     futs = [server.submit(serving.build_stream_request(sp, keep_material=True)) for sp in specs]
     got = [f.result(timeout=300) for f in futs]
     server.close()
