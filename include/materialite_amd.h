@@ -84,6 +84,13 @@ int mtl_solver_add_evp(mtl_solver* h, const double stiffness[36], int num_slip,
                        const double* schmid, double reference_slip_rate,
                        double rate_exponent, double slip_resistance, int hardening,
                        const double hardening_props[4], int* model_id);
+/* The same model with one initial slip resistance per slip system
+ * (slip_resistances: num_slip values), the per-system Scalar that
+ * elastic_viscoplastic.py:66-69 broadcasts onto every point. */
+int mtl_solver_add_evp_systems(mtl_solver* h, const double stiffness[36], int num_slip,
+                               const double* schmid, double reference_slip_rate,
+                               double rate_exponent, const double* slip_resistances,
+                               int hardening, const double hardening_props[4], int* model_id);
 
 /* IsotropicElasticPlastic (J2 radial return with isotropic hardening):
  * small_strain_fft/isotropic_elastic_plastic.py:17-192.  stiffness: the

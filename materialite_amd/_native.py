@@ -83,6 +83,10 @@ EXPORTS = {
                                           ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                                           ctypes.c_double, ctypes.c_int, ctypes.c_void_p,
                                           ctypes.POINTER(ctypes.c_int)]),
+    "mtl_solver_add_evp_systems": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                                  ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                                                  ctypes.c_void_p, ctypes.c_int, ctypes.c_void_p,
+                                                  ctypes.POINTER(ctypes.c_int)]),
     "mtl_solver_add_j2": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double,
                                          ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          ctypes.c_void_p, ctypes.POINTER(ctypes.c_int)]),
@@ -230,10 +234,18 @@ class NativeSolver:
         p = np.zeros(4)
         p[: len(props)] = props
         mid = ctypes.c_int()
-        check(self._lib.mtl_solver_add_evp(
-            self._h, _ptr(c), s.shape[0], _ptr(s), float(reference_slip_rate),
-            float(rate_exponent), float(slip_resistance), int(hardening), _ptr(p),
-            ctypes.byref(mid)))
+        g0 = np.asarray(slip_resistance, dtype=np.float64)
+        if g0.ndim == 0:
+            check(self._lib.mtl_solver_add_evp(
+                self._h, _ptr(c), s.shape[0], _ptr(s), float(reference_slip_rate),
+                float(rate_exponent), float(g0), int(hardening), _ptr(p), ctypes.byref(mid)))
+        else:
+            g0 = np.ascontiguousarray(g0.reshape(-1))
+            if g0.size != s.shape[0]:
+                raise ValueError(f"{g0.size} initial slip resistances for {s.shape[0]} slip systems")
+            check(self._lib.mtl_solver_add_evp_systems(
+                self._h, _ptr(c), s.shape[0], _ptr(s), float(reference_slip_rate),
+                float(rate_exponent), _ptr(g0), int(hardening), _ptr(p), ctypes.byref(mid)))
         return mid.value
 
     def add_j2(self, stiffness, shear_modulus, bulk_modulus, yield_stress, hardening, props):

@@ -23,12 +23,19 @@
 // (d) replaces NS read and NS written values per iteration; recomputing g_s
 // from it repeats the same operations and is bitwise identical.
 //
-// Slip resistances are one value per point, not per system: the model takes a
-// scalar initial slip resistance (elastic_viscoplastic.py:66-69 adds it to
-// every system), and each hardening law (hardening_laws.py:18-90) applies the
-// same operations to every system of a point (the increment depends on the
-// point's total slip only, Armstrong-Frederick's also on g_old_s itself), so a
-// point's NS values stay bitwise equal for the whole run.
+// Slip resistances are stored as one value g per point.  With one initial
+// slip resistance for every system (elastic_viscoplastic.py:66-69 adds a
+// scalar to every system) each hardening law (hardening_laws.py:18-90)
+// applies the same operations to every system of a point (the increment
+// depends on the point's total slip only, Armstrong-Frederick's also on
+// g_old_s itself), so a point's NS values stay bitwise equal for the whole
+// run.  With one initial value per system, g is that of a reference system r
+// and the others follow from it exactly: linear, Voce and perfect plasticity
+// add the same increment to every system, so g_s - g0_s = g - g0_r; the
+// Armstrong-Frederick step g' = (g + h d) / (1 + r d) scales every
+// (h/r - g_s) by the same factor, so g_s - g0_s = c_s (g - g0_r) with
+// c_s = (h/r - g0_s) / (h/r - g0_r) (r = 0 is the linear case).  Either way
+// g_s = g0_s + c_s (g - g0_r), which returns every g0_s exactly at the start.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -202,6 +209,9 @@ struct EvpParams {
   int nint;  // n-1 when it is an integer in [0, 64], else -1
   int hard;
   double hp[4];
+  int persys;  // slip resistances differ by system: g_s = g0_s + c_s (g - gref)
+  int pad;
+  double gref, g0s[NS], cs[NS];
 };
 
 // The model parameters (Schmid tensors, compliance: ~100 doubles) are staged in
@@ -330,16 +340,17 @@ __device__ __forceinline__ void evp_iterate(const EvpParams<NS>& q, const double
 #pragma unroll
   for (int k = 0; k < 21; ++k) S[k] = 0.0;
   double dsum = 0.0;
-  // one reciprocal for all systems (the slip resistance is the same for every
-  // system of a point) instead of two FP64 divisions per system; the rate
-  // differs from the divided form by a few ulps
+  // one reciprocal for all systems when the slip resistance is the same for
+  // every system of a point, instead of two FP64 divisions per system; the
+  // rate differs from the divided form by a few ulps
   const double ginv = 1.0 / gs;
 #pragma unroll 1
   for (int s = 0; s < NS; ++s) {
     double tau = 0.0;
 #pragma unroll
     for (int m = 0; m < 6; ++m) tau += q.P[s][m] * sig[m];
-    const double pref = q.gamma0 * powi_abs<IP>(tau * ginv, q.nint, q.n - 1.0) * ginv;
+    const double gi = q.persys ? 1.0 / (q.g0s[s] + q.cs[s] * (gs - q.gref)) : ginv;
+    const double pref = q.gamma0 * powi_abs<IP>(tau * gi, q.nint, q.n - 1.0) * gi;
     const double rate = pref * tau;
     const double w = q.n * pref;
 #pragma unroll
@@ -717,6 +728,7 @@ class EvpModel final : public Model<K> {
     q_.tol = kEvpTolerance;
     q_.hard = d.hardening;
     for (int k = 0; k < 4; ++k) q_.hp[k] = d.hp[k];
+    per_system_slip_resistances(d);
     const long long n = std::max(this->n_, 1);
     auto alloc = [&](double*& ptr, long long cnt) { MTL_HIP(hipMalloc(&ptr, cnt * sizeof(double))); };
     alloc(B_.e0, blk_elems(n, 6));
@@ -863,12 +875,16 @@ class EvpModel final : public Model<K> {
         return true;
       }
       case kFieldSlipResistance: {
-        // one value per point on the device, the same for every system
+        // one value per point on the device (that of every system, or of the
+        // reference system when they differ)
         fetch(slot_[ring_.tr].g, 1);
         std::vector<double> per_point(out);
         out.resize(size_t(NS) * n);
         for (int i = 0; i < n; ++i)
-          for (int k = 0; k < NS; ++k) out[size_t(i) * NS + k] = per_point[size_t(i)];
+          for (int k = 0; k < NS; ++k)
+            out[size_t(i) * NS + k] =
+                q_.persys ? q_.g0s[k] + q_.cs[k] * (per_point[size_t(i)] - q_.gref)
+                          : per_point[size_t(i)];
         ncomp = NS;
         return true;
       }
@@ -879,6 +895,32 @@ class EvpModel final : public Model<K> {
   }
 
  private:
+  // g0_s given per system: pick the reference system and the constants of
+  // g_s = g0_s + c_s (g - g0_r) (see the top of this file); d_.g0 becomes the
+  // reference system's initial value
+  void per_system_slip_resistances(const ModelDesc& d) {
+    q_.persys = 0;
+    if (d.g0_sys.empty()) return;
+    if (int(d.g0_sys.size()) != NS) throw Error("one initial slip resistance per slip system");
+    const std::vector<double>& g0 = d.g0_sys;
+    const bool af = d.hardening == kArmstrongFrederick && d.hp[1] != 0.0;
+    const double sat = af ? d.hp[0] / d.hp[1] : 0.0;
+    int ref = 0;
+    if (af)
+      for (int s = 1; s < NS; ++s)
+        if (std::fabs(sat - g0[s]) > std::fabs(sat - g0[ref])) ref = s;
+    d_.g0 = g0[ref];
+    bool uniform = true;
+    for (int s = 0; s < NS; ++s) uniform = uniform && g0[s] == g0[ref];
+    if (uniform) return;
+    q_.persys = 1;
+    q_.gref = g0[ref];
+    for (int s = 0; s < NS; ++s) {
+      q_.g0s[s] = g0[s];
+      q_.cs[s] = af && s != ref ? (sat - g0[s]) / (sat - g0[ref]) : 1.0;
+    }
+  }
+
   ModelDesc d_;
   Stiff21 C_;
   EvpParams<NS> q_{};

@@ -20,6 +20,7 @@ struct ModelDesc {
   int nslip = 0;
   std::vector<double> schmid;  // nslip*6, Mandel sym(d (x) n)
   double gamma0 = 0, n = 0, g0 = 0;  // g0: initial slip resistance / J2 yield stress
+  std::vector<double> g0_sys;        // nslip initial slip resistances (empty: g0 for all)
   int hardening = kPerfectPlasticity;
   double hp[4] = {};
   // J2: shear and bulk moduli

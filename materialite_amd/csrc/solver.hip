@@ -1051,10 +1051,11 @@ int mtl_solver_add_elastic(mtl_solver* h, const double stiffness[36], int* model
   });
 }
 
-int mtl_solver_add_evp(mtl_solver* h, const double stiffness[36], int num_slip,
-                       const double* schmid, double reference_slip_rate, double rate_exponent,
-                       double slip_resistance, int hardening, const double hardening_props[4],
-                       int* model_id) {
+namespace {
+int add_evp(mtl_solver* h, const double stiffness[36], int num_slip, const double* schmid,
+            double reference_slip_rate, double rate_exponent, double slip_resistance,
+            const double* per_system, int hardening, const double hardening_props[4],
+            int* model_id) {
   if (!h || !stiffness || !schmid || !model_id) return bad("null argument");
   if (hardening < 0 || hardening > 3) return bad("unknown hardening law");
   if (num_slip < 1) return bad("EVP model needs at least one slip system");
@@ -1067,10 +1068,29 @@ int mtl_solver_add_evp(mtl_solver* h, const double stiffness[36], int num_slip,
     d.gamma0 = reference_slip_rate;
     d.n = rate_exponent;
     d.g0 = slip_resistance;
+    if (per_system) d.g0_sys.assign(per_system, per_system + num_slip);
     d.hardening = hardening;
     if (hardening_props) std::memcpy(d.hp, hardening_props, sizeof(d.hp));
     *model_id = h->impl->add_model(d);
   });
+}
+}  // namespace
+
+int mtl_solver_add_evp(mtl_solver* h, const double stiffness[36], int num_slip,
+                       const double* schmid, double reference_slip_rate, double rate_exponent,
+                       double slip_resistance, int hardening, const double hardening_props[4],
+                       int* model_id) {
+  return add_evp(h, stiffness, num_slip, schmid, reference_slip_rate, rate_exponent,
+                 slip_resistance, nullptr, hardening, hardening_props, model_id);
+}
+
+int mtl_solver_add_evp_systems(mtl_solver* h, const double stiffness[36], int num_slip,
+                               const double* schmid, double reference_slip_rate,
+                               double rate_exponent, const double* slip_resistances,
+                               int hardening, const double hardening_props[4], int* model_id) {
+  if (!slip_resistances) return bad("null argument");
+  return add_evp(h, stiffness, num_slip, schmid, reference_slip_rate, rate_exponent,
+                 slip_resistances[0], slip_resistances, hardening, hardening_props, model_id);
 }
 
 int mtl_solver_add_j2(mtl_solver* h, const double stiffness[36], double shear_modulus,

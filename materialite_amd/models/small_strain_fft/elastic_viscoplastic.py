@@ -50,12 +50,34 @@ class ElasticViscoplastic(DeviceModel):
         self._state_variable_info = dict(zip(self.available_state_variables,
                                              [(Scalar, n), (Order2SymmetricTensor, 0), (Scalar, n)]))
 
+    def _initial_slip_resistances(self):
+        """A float, or one value per slip system: the initial slip resistance
+        is added to a (points, systems) zero Scalar (elastic_viscoplastic.py:
+        66-69), so a number or a Scalar over systems ("s") broadcasts onto
+        every point; a per-point Scalar is not supported."""
+        g0 = self.initial_slip_resistance
+        n = len(self.slip_systems)
+        if isinstance(g0, Scalar):
+            if g0.dims_str == "":
+                return float(g0.components)
+            if g0.dims_str != "s":
+                raise NotImplementedError(
+                    f"initial slip resistance with dims {g0.dims_str!r}: only a number or "
+                    "one value per slip system (dims 's') is supported")
+            g0 = g0.components
+        g0 = np.asarray(g0, dtype=np.float64)
+        if g0.ndim == 0:
+            return float(g0)
+        if g0.shape != (n,):
+            raise ValueError(f"initial slip resistance of shape {g0.shape}: expected ({n},)")
+        return g0
+
     def _add_to(self, native):
         kind, props = self.kernel_hardening()
         return native.add_evp(self.stiffness.components, self.slip_systems.schmid_tensor.components,
                               float(self.reference_slip_rate.components),
                               float(self.rate_exponent.components),
-                              float(self.initial_slip_resistance), kind, props)
+                              self._initial_slip_resistances(), kind, props)
 
     def initialize(self, orientations):
         """elastic_viscoplastic.py:52-89: zero plastic state, slip resistances at

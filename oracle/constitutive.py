@@ -111,7 +111,9 @@ class ElasticViscoplastic:
         self.schmid = np.asarray(schmid, float)  # (S, 6) Mandel sym(d n)
         self.gamma0 = float(reference_slip_rate)
         self.n = float(rate_exponent)
-        self.g0 = float(slip_resistance)
+        # a number or one value per slip system, broadcast onto every point
+        # (elastic_viscoplastic.py:66-69 adds it to a (points, systems) zero)
+        self.g0 = np.asarray(slip_resistance, float)
         self.hardening = HARDENING[hardening] if isinstance(hardening, str) else hardening
         self.props = hardening_properties
         self._pp = np.einsum("sm,sn->smn", self.schmid, self.schmid)
@@ -123,7 +125,7 @@ class ElasticViscoplastic:
         n_sys = self.schmid.shape[0]
         self.R = np.asarray(rotations, float)
         self.Rm = rotation_matrix_mandel(self.R)
-        g = np.full((n_pts, n_sys), self.g0)
+        g = np.zeros((n_pts, n_sys)) + self.g0
         self.g = g.copy()
         self.g_old = g.copy()
         self.ep = np.zeros((n_pts, 6))  # crystal frame, trial
