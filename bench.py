@@ -167,6 +167,11 @@ def serving_stream(nreq, rate, slots, batching=False, sizes=(32, 48, 64, 96), la
     return summ
 
 
+# bytes per stored real value: tangent and MINRES w / x vectors, Lanczos
+# vectors, spectrum (refined: an FP32 value plus a bf16 remainder)
+WIDTHS = {"fp64": (8, 8, 8), "mixed": (4, 4, 4), "refined": (4, 6, 6)}
+
+
 def serving_report(nreq, slots):
     """The mixed-size stream (32-96^3) batched and unbatched at a rate the
     server sustains and at one above it, and a stream of 32^3-48^3 requests
