@@ -92,6 +92,10 @@ def parse():
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="MiB copied by a plain device copy before the workload (the reference "
                          "dispatch for scaling rocprofv3 FETCH_SIZE/WRITE_SIZE)")
+    ap.add_argument("--flops-probe", type=int, default=0,
+                    help="Mi FP64 elements multiplied by a plain device kernel before the workload "
+                         "(one FP64 multiply each: the reference dispatch for scaling rocprofv3 "
+                         "SQ_INSTS_VALU_FLOPS_FP64)")
     ap.add_argument("--summarize-pmc", nargs=2, metavar=("FETCH_CSV", "WRITE_CSV"),
                     help="print the per-dispatch HBM traffic summary of two rocprofv3 --pmc runs")
     ap.add_argument("--summarize-valu", metavar="COUNTER_CSV",
@@ -413,7 +417,6 @@ KERNEL_SYMBOLS = {"z_forward": r"k_z_forward<double, mtl::f48, 1,", "y_pass": r"
                   "x_ghat": r"k_x_ghat_\w+<double, mtl::f48", "z_inverse": r"k_z_inverse<double, mtl::f48, 1,",
                   "minres_vector": r"k_(lanczos|update)<mtl::f48", "evp_iteration": r"k_evp_iter<12, true"}
 FP64_PEAK_TFLOPS = 78.6
-NUM_CUS = 256
 
 
 def summarize_pmc(fetch_csv, write_csv):
@@ -550,11 +553,12 @@ def measured_valu(kernel):
     return None if cells is None else (float(cells[2]), float(cells[3]), float(cells[4]))
 
 
-def print_valu_summary(csv_path):
+def print_valu_summary(csv_path, probe_mi):
     """Per working dispatch of the EVP iteration kernel: FP64 FLOPs
-    (SQ_INSTS_VALU_FLOPS_FP64), VALU busy (SQ_ACTIVE_INST_VALU over
-    GRBM_GUI_ACTIVE and the CU count) and VALU lane utilisation
-    (SQ_THREAD_CYCLES_VALU over 64 x SQ_ACTIVE_INST_VALU)."""
+    (SQ_INSTS_VALU_FLOPS_FP64, scaled so that the FP64 multiply probe of
+    probe_mi Mi elements counts one FLOP per element), and rocprofv3's derived
+    VALUBusy (VALU instruction cycles per CU over the busiest XCD's active
+    cycles) and VALUUtilization (active lanes per VALU instruction)."""
     import csv
     import re
     from collections import defaultdict
@@ -565,7 +569,14 @@ def print_valu_summary(csv_path):
         key = r.get("Dispatch_Id") or r.get("Correlation_Id")
         per[key][r["Counter_Name"]] += float(r["Counter_Value"])
         names[key] = r["Kernel_Name"]
-    print("| kernel | working dispatches | FP64 FLOPs / dispatch | VALU busy % | VALU lane utilisation % |")
+    probes = [v.get("SQ_INSTS_VALU_FLOPS_FP64", 0.0) for k, v in per.items()
+              if "elementwise" in names[k]]
+    scale = 1.0
+    if probes and max(probes) > 0:
+        scale = probe_mi * 2 ** 20 / max(probes)
+        print(f"FP64 multiply probe: {probe_mi} Mi elements (one FP64 multiply each); "
+              f"SQ_INSTS_VALU_FLOPS_FP64 read {max(probes):.6g}; scale factor {scale:.4f}\n")
+    print("| kernel | working dispatches | FP64 FLOPs / dispatch (scaled) | VALU busy % | VALU lane utilisation % |")
     print("|---|---:|---:|---:|---:|")
     for name in ("evp_iteration",):
         rows = [v for k, v in per.items() if re.search(KERNEL_SYMBOLS[name], names[k])]
@@ -573,11 +584,9 @@ def print_valu_summary(csv_path):
             continue
         top = max(r.get("SQ_INSTS_VALU_FLOPS_FP64", 0.0) for r in rows)
         rows = [r for r in rows if r.get("SQ_INSTS_VALU_FLOPS_FP64", 0.0) > 0.2 * top]
-        flops = sum(r["SQ_INSTS_VALU_FLOPS_FP64"] for r in rows) / len(rows)
-        busy = sum(100.0 * r["SQ_ACTIVE_INST_VALU"] / NUM_CUS / max(r["GRBM_GUI_ACTIVE"], 1.0)
-                   for r in rows) / len(rows)
-        util = sum(100.0 * r["SQ_THREAD_CYCLES_VALU"] / (64.0 * max(r["SQ_ACTIVE_INST_VALU"], 1.0))
-                   for r in rows) / len(rows)
+        flops = scale * sum(r["SQ_INSTS_VALU_FLOPS_FP64"] for r in rows) / len(rows)
+        busy = sum(r["VALUBusy"] for r in rows) / len(rows)
+        util = sum(r["VALUUtilization"] for r in rows) / len(rows)
         print(f"| {name} | {len(rows)} | {flops:.4g} | {busy:.1f} | {util:.1f} |")
 
 
@@ -792,7 +801,7 @@ def main():
         print_trace_summary(args.summarize_trace)
         return
     if args.summarize_valu:
-        print_valu_summary(args.summarize_valu)
+        print_valu_summary(args.summarize_valu, args.flops_probe or 64)
         return
     if args.summarize_pmc:
         print_pmc_summary(args.summarize_pmc[0], args.summarize_pmc[1], args.copy_probe or 512)
@@ -816,6 +825,14 @@ def main():
             torch.mul(src, 1.0, out=dst)  # one elementwise kernel: reads and writes the size
         torch.cuda.synchronize()
         del src, dst
+    if args.flops_probe:
+        a = torch.full((args.flops_probe * 2 ** 20,), 1.5, dtype=torch.float64, device="cuda")
+        b = torch.full_like(a, 0.75)
+        c = torch.empty_like(a)
+        for _ in range(3):
+            torch.mul(a, b, out=c)  # one elementwise kernel: one FP64 multiply per element
+        torch.cuda.synchronize()
+        del a, b, c
     n = args.grid
     npts = n ** 3
     material = build_material(n, args.grains)
