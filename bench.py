@@ -60,6 +60,13 @@ HBM_PEAK_GBS = 8000.0
 SEED = 20260003
 
 
+def _plain(o):
+    """JSON form of NumPy scalars and arrays in the bench line."""
+    if hasattr(o, "tolist"):
+        return o.tolist()
+    raise TypeError(f"{type(o).__name__} in the bench line")
+
+
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -155,7 +162,7 @@ def serving_stream(nreq, rate, slots, batching=False, sizes=(32, 48, 64, 96), la
     span = max(r.finished for r in reqs) - min(r.arrival for r in reqs)
     summ.update({
         "arrival_rate_per_s": rate,
-        "sustained": summ["solves_per_s"] >= 0.95 * rate,
+        "sustained": bool(summ["solves_per_s"] >= 0.95 * rate),
         "batching": batching,
         "lanes_per_size": lanes if batching else None,
         "slots": slots,
@@ -816,7 +823,8 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     if args.stream_only:
         print(json.dumps({"serving": serving_stream(args.stream_requests, args.stream_rate,
-                                                    args.stream_slots, batching=args.batching)}))
+                                                    args.stream_slots, batching=args.batching)},
+                         default=_plain))
         return
     if args.copy_probe:
         src = torch.ones(args.copy_probe * 2 ** 18, dtype=torch.float32, device="cuda")
@@ -993,7 +1001,7 @@ def main():
         "cpu_baseline": cpu,
         "config1_16": c1,
     }
-    print(json.dumps(line))
+    print(json.dumps(line, default=_plain))
 
 
 if __name__ == "__main__":

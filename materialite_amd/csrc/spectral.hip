@@ -274,7 +274,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
       for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
         const int cxp = i / hp;  // c*nx + x
         const int t = 2 * (i - cxp * hp);
-        const int c = cxp / nx, x = cxp - c * nx;
+        const int x = cxp / 6, c = cxp - x * 6;
         base.get_pair(c * g.nspec + x * xstride + t, buf + x * (6 * TX) + c * TX + t);
       }
     }
@@ -283,7 +283,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
     for (int i = threadIdx.x; i < tot; i += kXBT) {
       const int t = i % TX;
       const int cxp = i / TX;  // c*nx + x
-      const int c = cxp / nx, x = cxp - c * nx;
+      const int x = cxp / 6, c = cxp - x * 6;
       buf[x * (6 * TX) + c * TX + t] =
           t < width ? base.template get<T>(c * g.nspec + x * xstride + t) : cplx<T>{T(0), T(0)};
     }
@@ -324,7 +324,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
       for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
         const int cxp = i / hp;
         const int t = 2 * (i - cxp * hp);
-        const int c = cxp / nx, x = cxp - c * nx;
+        const int x = cxp / 6, c = cxp - x * 6;
         base.put_pair(c * g.nspec + x * xstride + t, out + x * (6 * TX) + c * TX + t);
       }
     }
@@ -333,7 +333,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
     for (int i = threadIdx.x; i < tot; i += kXBT) {
       const int t = i % TX;
       const int cxp = i / TX;
-      const int c = cxp / nx, x = cxp - c * nx;
+      const int x = cxp / 6, c = cxp - x * 6;
       if (t < width) base.put(c * g.nspec + x * xstride + t, out[x * (6 * TX) + c * TX + t]);
     }
   }
