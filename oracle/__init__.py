@@ -23,9 +23,14 @@ Pinning (how each function is checked, see ``tests/test_oracle_*.py``):
                     150.3980086, 171.694205 and the reference's step relations
                     between axial stress and equivalent plastic strain; J2
                     two-phase 150.3980086).
+                    Per-system initial slip resistances: a system whose
+                    resistance is effectively infinite never slips, so the model
+                    equals the one without those systems; equal per-system values
+                    equal the scalar model.
 * ``solver``        end-to-end reference goldens (elastic: 6.5890252, 6.0,
                     4.547568053, homogeneous closed forms, periodic-shift
-                    invariance).
+                    invariance); the load-schedule factories against the
+                    reference's known answers (tests/test_load_schedule.py).
 * ``microstructure`` reference known answers for seeded Voronoi region ids and
                     Haar-random orientations.
 """

@@ -210,3 +210,35 @@ def test_hcp_slip_systems_known_answers():
                       (1 + r3) / 6)
     assert np.isclose(T.max_schmid_factor(*T.hcp_slip_systems(["pyramidal_ca"], 1.5), l111),
                       (7 + 5 * r3) / (12 * np.sqrt(13)))
+
+
+def test_oracle_periodic_shift_gives_shifted_stress():
+    """tests/test_elastic_model.py:225-270: a microstructure shifted
+    periodically gives the same stress field, shifted.  The oracle's solver on
+    a 6^3 grid of 8 grains (random orientations), one uniaxial-z increment,
+    shifted by (1, 3, 2) voxels."""
+    from oracle import constitutive as C
+    from oracle import tensors as T
+    from oracle.solver import LoadSchedule as OLS
+    from oracle.solver import SmallStrainFFT as OSolver
+    from tests import fixtures as F
+
+    n = 6
+    rng = np.random.default_rng(21)
+    grains = rng.integers(0, 8, size=(n, n, n))
+    rot_g = T.rotation_from_euler(T.random_euler(8, rng))
+
+    def solve(g):
+        rot = rot_g[g.ravel()]
+        out = OSolver(OLS.from_constant_uniaxial_strain_rate(direction="z"), end_time=1.0,
+                      initial_time_increment=1.0).run([n] * 3, rot, C.Elastic(F.TI_STIFFNESS),
+                                                      global_tolerance=1e-8,
+                                                      linear_solver_tolerance=1e-10)
+        return out["stress"].reshape(n, n, n, 6)
+
+    shift = (1, 3, 2)
+    base = solve(grains)
+    moved = solve(np.roll(grains, shift, axis=(0, 1, 2)))
+    want = np.roll(base, shift, axis=(0, 1, 2))
+    assert np.abs(moved - want).max() <= 1e-9 * np.abs(base).max()
+    assert np.abs(base - base.mean(axis=(0, 1, 2))).max() > 1e-3 * np.abs(base).max()
