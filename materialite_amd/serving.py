@@ -281,19 +281,24 @@ class RveServer:
 
 
 def latency_summary(requests):
-    """p50/p99 latency (s), mean queueing delay and solves/s over finished requests."""
+    """p50/p99 latency (s), mean queueing delay, solves/s over finished
+    requests (first arrival to last completion) and the offered rate (arrivals
+    per second between the first and the last arrival)."""
     done = [r for r in requests if r.finished is not None and r.error is None]
     if not done:
         return {"count": 0}
     lat = np.array([r.latency for r in done])
     wait = np.array([r.admitted - r.arrival for r in done])
     span = max(r.finished for r in done) - min(r.arrival for r in done)
+    arrivals = sorted(r.arrival for r in requests)
+    window = arrivals[-1] - arrivals[0]
     return {
         "count": len(done),
         "p50_s": float(np.percentile(lat, 50)),
         "p99_s": float(np.percentile(lat, 99)),
         "mean_queue_s": float(wait.mean()),
         "solves_per_s": len(done) / span if span > 0 else float("inf"),
+        "offered_rate_per_s": (len(arrivals) - 1) / window if window > 0 else float("inf"),
     }
 
 
