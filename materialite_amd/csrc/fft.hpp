@@ -372,6 +372,94 @@ __device__ __forceinline__ cplx<T>* block_fft(cplx<T>* a, cplx<T>* b, int M, int
   return a;
 }
 
+// In-place form of block_fft for one buffer: every pass first loads the
+// inputs of all of a thread's butterflies (with their twiddles) into
+// registers, then a barrier, then computes them and writes the outputs to
+// their Stockham positions in the same buffer, then a barrier.  It needs half
+// the LDS of the ping-pong form, so a workgroup can hold twice as many lines.
+// K bounds the complex values a thread holds: a radix-R pass handles up to K/R
+// butterflies per thread (butterfly b = j * M + m of thread t is t + k * BT,
+// as in block_fft), which inplace_fits() checks on the host.  Radices 2, 3, 4,
+// 5, 7, 8 and 16 only (no BIG radices, no generic pass).
+template <typename T, int BT, int K, int R>
+__device__ __forceinline__ void inplace_pass(cplx<T>* a, int M, int S, int per_line, int Ns,
+                                             unsigned long long mg, int tstride,
+                                             const cplx<T>* tw, bool inv) {
+  constexpr int NB = K / R;
+  const int total = M * per_line;
+  cplx<T> v[NB][R];
+  int base[NB], line[NB];
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    const int b = int(threadIdx.x) + k * BT;
+    base[k] = -1;
+    if (b < total) {
+      const int j = b / M, m = b - j * M;
+      const int jq = fast_div(j, mg);
+      const int kk = j - jq * Ns;
+#pragma unroll
+      for (int t = 0; t < R; ++t) {
+        cplx<T> x = a[(j + t * per_line) * S + m];
+        if (t > 0 && kk > 0) {
+          cplx<T> w = tw[t * kk * tstride];
+          if (inv) w.y = -w.y;
+          x = cmul(x, w);
+        }
+        v[k][t] = x;
+      }
+      base[k] = jq * Ns * R + kk;
+      line[k] = m;
+    }
+  }
+  lds_barrier();
+#pragma unroll
+  for (int k = 0; k < NB; ++k) {
+    if (base[k] >= 0) {
+      Dft<T, R>::run(v[k], inv, tw, per_line);
+#pragma unroll
+      for (int u = 0; u < R; ++u) a[(base[k] + u * Ns) * S + line[k]] = v[k][u];
+    }
+  }
+  lds_barrier();
+}
+
+template <typename T, int BT, int K>
+__device__ __forceinline__ void block_fft_inplace(cplx<T>* a, int M, int S,
+                                                  const AxisPlan* __restrict__ pl,
+                                                  const cplx<T>* tw, bool inv) {
+  const int npass = pl->npass;
+  const int L = pl->L;
+  for (int p = 0; p < npass; ++p) {
+    const int R = pl->radix[p];
+    const int Ns = pl->ns[p];
+    const unsigned long long mg = pl->ns_magic[p];
+    const int per_line = L / R;
+    const int tstride = per_line / Ns;
+    switch (R) {
+      case 16: inplace_pass<T, BT, K, 16>(a, M, S, per_line, Ns, mg, tstride, tw, inv); break;
+      case 8: inplace_pass<T, BT, K, 8>(a, M, S, per_line, Ns, mg, tstride, tw, inv); break;
+      case 4: inplace_pass<T, BT, K, 4>(a, M, S, per_line, Ns, mg, tstride, tw, inv); break;
+      case 2: inplace_pass<T, BT, K, 2>(a, M, S, per_line, Ns, mg, tstride, tw, inv); break;
+      case 3: inplace_pass<T, BT, K, 3>(a, M, S, per_line, Ns, mg, tstride, tw, inv); break;
+      case 5: inplace_pass<T, BT, K, 5>(a, M, S, per_line, Ns, mg, tstride, tw, inv); break;
+      case 7: inplace_pass<T, BT, K, 7>(a, M, S, per_line, Ns, mg, tstride, tw, inv); break;
+      default: break;  // rejected on the host (inplace_fits)
+    }
+  }
+}
+
+// Host: whether block_fft_inplace<T, BT, K> can run M lines of plan p.
+inline bool inplace_fits(const AxisPlan& p, int M, int BT, int K) {
+  for (int i = 0; i < p.npass; ++i) {
+    const int R = p.radix[i];
+    if (R != 2 && R != 3 && R != 4 && R != 5 && R != 7 && R != 8 && R != 16) return false;
+    if (K / R < 1) return false;
+    const long long butterflies = (long long)M * (p.L / R);
+    if (butterflies > (long long)(K / R) * BT) return false;
+  }
+  return true;
+}
+
 // Host: factor L into radices 16,8,4,2,3,5,7,11,13 (largest powers of two
 // first), then into the remaining primes, which run the generic pass.
 // uses_big_radix(): the plan needs the BIG block_fft (radix 11, 13 or generic).

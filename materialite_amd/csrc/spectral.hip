@@ -37,6 +37,8 @@ constexpr size_t kXPairBytes = 40 * 1024;
 // these sizes beat 32-96 KB tiles by 25-40 %.
 constexpr size_t kZBufBytes = 16 * 1024, kYBufBytes = 16 * 1024, kXBufBytes = 24 * 1024;
 constexpr size_t kMaxLds = 160 * 1024;
+// LDS of an in-place x-pass tile: three workgroups per CU
+constexpr size_t kXInplaceBytes = 53 * 1024;
 
 // LDS row stride of the z passes: 3 interleaved lines (paired components) per
 // grid line, rounded up to odd
@@ -237,8 +239,16 @@ __device__ __forceinline__ double rnorm3<double>(double a, double b, double c) {
 // Tile = (y, TX consecutive kz columns) with all six components resident in
 // LDS: forward x-FFT of the 6 TX interleaved lines, Ghat on each point, inverse
 // x-FFT, one HBM read and one write of the tile.  Used where TX >= 2 fits the
-// budget (grids up to about 128 points along x).
-template <typename T, typename S, bool BIG>
+// budget (grids up to about 128 points along x).  INPLACE runs the FFT passes
+// in the tile's one buffer (block_fft_inplace), which allows twice the tile
+// width for the same LDS: each (component, x) row of the tile is then a wider
+// contiguous run in HBM, and the pass is bound by the number of such rows.
+template <typename T>
+constexpr int x_inplace_regs() {
+  return sizeof(T) == 8 ? 16 : 32;  // complex values a thread holds per pass
+}
+
+template <typename T, typename S, bool BIG, bool INPLACE = false>
 __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
                                                   int ntiles, const T* tanx, const T* tany,
                                                   const T* tanz, int maskbits,
@@ -272,7 +282,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   if constexpr (CVec<S>::kPairs) {
     if (pairs) {
       for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
-        const int cxp = i / hp;  // c*nx + x
+        const int cxp = i / hp;  // x*6 + c
         const int t = 2 * (i - cxp * hp);
         const int x = cxp / 6, c = cxp - x * 6;
         base.get_pair(c * g.nspec + x * xstride + t, buf + x * (6 * TX) + c * TX + t);
@@ -282,7 +292,7 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   if (!pairs) {
     for (int i = threadIdx.x; i < tot; i += kXBT) {
       const int t = i % TX;
-      const int cxp = i / TX;  // c*nx + x
+      const int cxp = i / TX;  // x*6 + c
       const int x = cxp / 6, c = cxp - x * 6;
       buf[x * (6 * TX) + c * TX + t] =
           t < width ? base.template get<T>(c * g.nspec + x * xstride + t) : cplx<T>{T(0), T(0)};
@@ -293,8 +303,12 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
   const int nyqy = (g.ny % 2 == 0) ? g.ny / 2 : -1;
   const int nyqz = (g.nz % 2 == 0) ? g.nz / 2 : -1;  // never stored, kept for clarity
   const T ty = tany[y];
-  cplx<T>* buf2 = buf + 6 * TX * nx;
-  cplx<T>* res = block_fft<T, kXBT, BIG>(buf, buf2, 6 * TX, 6 * TX, plan, tw, false);
+  cplx<T>* buf2 = buf + 6 * TX * nx;  // the second LDS buffer (ping-pong form only)
+  cplx<T>* res = buf;
+  if constexpr (INPLACE)
+    block_fft_inplace<T, kXBT, x_inplace_regs<T>()>(buf, 6 * TX, 6 * TX, plan, tw, false);
+  else
+    res = block_fft<T, kXBT, BIG>(buf, buf2, 6 * TX, 6 * TX, plan, tw, false);
   for (int i = threadIdx.x; i < nx * TX; i += kXBT) {
     const int t = i % TX, kx = i / TX;
     const int kz = kz0 + t;
@@ -317,8 +331,11 @@ __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan
     for (int c = 0; c < 6; ++c) res[kx * (6 * TX) + c * TX + t] = X[c];
   }
   __syncthreads();
-  const cplx<T>* out =
-      block_fft<T, kXBT, BIG>(res, res == buf ? buf2 : buf, 6 * TX, 6 * TX, plan, tw, true);
+  const cplx<T>* out = res;
+  if constexpr (INPLACE)
+    block_fft_inplace<T, kXBT, x_inplace_regs<T>()>(res, 6 * TX, 6 * TX, plan, tw, true);
+  else
+    out = block_fft<T, kXBT, BIG>(res, res == buf ? buf2 : buf, 6 * TX, 6 * TX, plan, tw, true);
   if constexpr (CVec<S>::kPairs) {
     if (pairs) {
       for (int i = threadIdx.x; i < tot / 2; i += kXBT) {
@@ -679,6 +696,21 @@ void Spectral<S, A>::plan(int nx, int ny, int nz) {
   // single-component buffers fit
   tx_ = int(std::min<size_t>({8, size_t(g_.nzc), xbudget / (2 * 6 * nx * cb)}));
   x_resident_ = tx_ >= std::min(2, g_.nzc);
+  // in-place FFT passes: the same LDS holds twice the tile width (up to 8
+  // columns, three 53 KB workgroups per CU) where every pass fits the
+  // registers a thread holds
+  x_inplace_ = false;
+  if (x_resident_ && !big_x_) {
+    for (int t = 8; t > tx_; t /= 2) {
+      const int tw = std::min(t, g_.nzc);
+      if ((size_t(nx) + 6 * size_t(tw) * nx) * cb <= kXInplaceBytes &&
+          inplace_fits(px_, 6 * tw, kXBT, x_inplace_regs<A>())) {
+        tx_ = tw;
+        x_inplace_ = true;
+        break;
+      }
+    }
+  }
   if (!x_resident_) {
     tx_ = 0;
     for (int t = int(128 / cb); t >= 1; t /= 2)
@@ -816,10 +848,13 @@ void Spectral<S, A>::run(In in, const W* tangent, double* out, RVec<S> r1, const
 
   const int xtiles = ceil_div(g_.nzc, tx_);
   const size_t xlds =
-      x_resident_ ? (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<A>)
+      x_inplace_  ? (size_t(g_.nx) + size_t(6 * tx_) * g_.nx) * sizeof(cplx<A>)
+      : x_resident_ ? (size_t(g_.nx) + 2 * size_t(6 * tx_) * g_.nx) * sizeof(cplx<A>)
       : x_pairs_  ? (size_t(g_.nx) + 2 * size_t(tx_) * g_.nx) * sizeof(cplx<A>)
                   : x_lds_bytes(g_.nx, tx_, sizeof(cplx<A>));
-  auto kx = big_x_ ? k_x_ghat_resident<A, S, true> : k_x_ghat_resident<A, S, false>;
+  auto kx = big_x_     ? k_x_ghat_resident<A, S, true>
+            : x_inplace_ ? k_x_ghat_resident<A, S, false, true>
+                         : k_x_ghat_resident<A, S, false>;
   if (!x_resident_) {
     if constexpr (std::is_same<S, float>::value && sizeof(A) == 4) {
       if (x_pairs_)

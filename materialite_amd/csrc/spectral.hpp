@@ -92,6 +92,7 @@ class Spectral {
   int tx_;        // kz columns per workgroup in the x pass
   bool x_resident_;  // x pass keeps all six components of its tile in LDS
   bool x_pairs_ = false;  // streamed x pass moves pairs of complex values
+  bool x_inplace_ = false;  // resident x pass with in-place (one-buffer) FFT passes
   bool big_x_ = false, big_y_ = false, big_z_ = false;  // axis plans use radix 11 or 13
   void* spec_mem_ = nullptr;
   CVec<S> spec_;

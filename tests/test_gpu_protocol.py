@@ -170,11 +170,14 @@ def test_multiphase_protocol_matches_oracle():
     assert _rel(d_dev.components, d_ref) <= 1e-10
     dev.update_state_variables()
     ref.update_state_variables()
-    out = dev.postprocess(["slip_resistances", "plastic_strains"])
+    out = dev.postprocess(["slip_resistances", "plastic_strains", "slip_system_shear_strains"])
     g = out["slip_resistances"].components
     assert _rel(g[idx[0]], evp_ref.g) <= 1e-10
     assert np.all(g[idx[1]] == 0.0)
     assert _rel(out["plastic_strains"].components[idx[0]], evp_ref.ep_specimen) <= 1e-9
+    # the EVP model's points do not fill whole 64-point blocks of its state
+    assert len(idx[0]) % 64 != 0
+    assert _rel(out["slip_system_shear_strains"].components[idx[0]], evp_ref.gamma) <= 1e-9
     with pytest.raises(ValueError, match="not available"):
         Multiphase([1], [hole], [np.arange(512)], 512).generate_outputs(["slip_resistances"])
 
