@@ -39,6 +39,8 @@ constexpr size_t kZBufBytes = 16 * 1024, kYBufBytes = 16 * 1024, kXBufBytes = 24
 constexpr size_t kMaxLds = 160 * 1024;
 // LDS of an in-place x-pass tile: three workgroups per CU
 constexpr size_t kXInplaceBytes = 53 * 1024;
+// LDS of an in-place y-pass tile: four workgroups per CU
+constexpr size_t kYInplaceBytes = 40 * 1024;
 
 // LDS row stride of the z passes: 3 interleaved lines (paired components) per
 // grid line, rounded up to odd
@@ -143,8 +145,16 @@ __global__ void __launch_bounds__(kZBT) k_z_forward(Grid g, const AxisPlan* plan
   }
 }
 
+// complex values a thread holds per in-place FFT pass (block_fft_inplace)
+template <typename T>
+constexpr int x_inplace_regs() {
+  return sizeof(T) == 8 ? 16 : 32;
+}
+
 // ----------------------------------------------------------------- y pass
-template <typename T, typename S, bool BIG>
+// INPLACE: the FFT passes run in the tile's one LDS buffer
+// (block_fft_inplace), which allows twice the tile width for the same LDS.
+template <typename T, typename S, bool BIG, bool INPLACE = false>
 __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g, const cplx<T>* twy, int TY,
                                                   int ntiles, bool inv,
                                                   const MinresState* st, CVec<S> spec,
@@ -186,7 +196,11 @@ __global__ void __launch_bounds__(kYBT) k_y_pass(Grid g, const AxisPlan* plan_g,
     }
   }
   __syncthreads();
-  const cplx<T>* res = block_fft<T, kYBT, BIG>(buf, buf + TY * ny, TY, TY, plan, tw, inv);
+  const cplx<T>* res = buf;
+  if constexpr (INPLACE)
+    block_fft_inplace<T, kYBT, x_inplace_regs<T>()>(buf, TY, TY, plan, tw, inv);
+  else
+    res = block_fft<T, kYBT, BIG>(buf, buf + TY * ny, TY, TY, plan, tw, inv);
   if constexpr (CVec<S>::kPairs) {
     if (pairs) {
       for (int i = threadIdx.x; i < hp * ny; i += kYBT) {
@@ -243,10 +257,6 @@ __device__ __forceinline__ double rnorm3<double>(double a, double b, double c) {
 // in the tile's one buffer (block_fft_inplace), which allows twice the tile
 // width for the same LDS: each (component, x) row of the tile is then a wider
 // contiguous run in HBM, and the pass is bound by the number of such rows.
-template <typename T>
-constexpr int x_inplace_regs() {
-  return sizeof(T) == 8 ? 16 : 32;  // complex values a thread holds per pass
-}
 
 template <typename T, typename S, bool BIG, bool INPLACE = false>
 __global__ void __launch_bounds__(kXBT) k_x_ghat_resident(Grid g, const AxisPlan* plan_g, const cplx<T>* twx, int TX,
@@ -690,6 +700,22 @@ void Spectral<S, A>::plan(int nx, int ny, int nz) {
   // y pass: ty_ kz columns per workgroup
   if ((size_t(ny) + 2 * size_t(ny)) * cb > kMaxLds) throw Error("ny too large for the y pass");
   ty_ = int(std::max<size_t>(1, std::min<size_t>({16, size_t(g_.nzc), ybudget / (2 * ny * cb)})));
+  // FP64 arithmetic: in-place FFT passes, twice the tile width (up to 32
+  // columns) in at most 40 KB (at 128^3: 46.7 -> 43.9 us refined, 54.5 -> 53.4
+  // us FP64; with FP32 arithmetic the register-staged passes measured slower,
+  // 25.3 -> 28.1 us, so the FP32 y pass keeps the ping-pong form)
+  y_inplace_ = false;
+  if (!big_y_ && sizeof(A) == 8) {
+    for (int t = 32; t > ty_; t /= 2) {
+      const int tw = std::min(t, g_.nzc);
+      if ((size_t(ny) + size_t(tw) * ny) * cb <= kYInplaceBytes &&
+          inplace_fits(py_, tw, kYBT, x_inplace_regs<A>())) {
+        ty_ = tw;
+        y_inplace_ = true;
+        break;
+      }
+    }
+  }
   // x pass: tx_ kz columns of all 6 components resident per workgroup where
   // at least two fit the budget; otherwise the components stream through LDS
   // one at a time with tx_ the widest (up to 128 B rows) tile whose two
@@ -840,8 +866,10 @@ void Spectral<S, A>::run(In in, const W* tangent, double* out, RVec<S> r1, const
 
   const MinresState* stp = MODE == kModeMatvec ? st : nullptr;
   const int ytiles = ceil_div(g_.nzc, ty_);
-  const size_t ylds = (size_t(g_.ny) + 2 * size_t(ty_) * g_.ny) * sizeof(cplx<A>);
-  auto ky = big_y_ ? k_y_pass<A, S, true> : k_y_pass<A, S, false>;
+  const size_t ylds = (size_t(g_.ny) + (y_inplace_ ? 1 : 2) * size_t(ty_) * g_.ny) * sizeof(cplx<A>);
+  auto ky = big_y_       ? k_y_pass<A, S, true>
+            : y_inplace_ ? k_y_pass<A, S, false, true>
+                         : k_y_pass<A, S, false>;
   allow_lds(ky, ylds);
   profiler().launch(MODE == kModeMatvec ? kKYPass : residual_id(kKYPass), ky, dim3(6 * g_.nx * ytiles, nb), dim3(kYBT), ylds, s, g_, pyd_, twy_, ty_, ytiles, false, stp, spec_, slots);
   MTL_LAUNCH_CHECK();

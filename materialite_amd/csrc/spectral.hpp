@@ -93,6 +93,7 @@ class Spectral {
   bool x_resident_;  // x pass keeps all six components of its tile in LDS
   bool x_pairs_ = false;  // streamed x pass moves pairs of complex values
   bool x_inplace_ = false;  // resident x pass with in-place (one-buffer) FFT passes
+  bool y_inplace_ = false;  // y pass with in-place FFT passes
   bool big_x_ = false, big_y_ = false, big_z_ = false;  // axis plans use radix 11 or 13
   void* spec_mem_ = nullptr;
   CVec<S> spec_;
